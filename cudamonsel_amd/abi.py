@@ -1,0 +1,122 @@
+"""ctypes declarations of the C-ABI (include/monsel_gpu.h) for the parts the
+pybind host API does not cover: direct job submission and the probes."""
+import ctypes
+
+import numpy as np
+
+from . import gpu_lib, last_error
+
+c_float_p = ctypes.POINTER(ctypes.c_float)
+c_int_p = ctypes.POINTER(ctypes.c_int)
+c_double_p = ctypes.POINTER(ctypes.c_double)
+c_u64_p = ctypes.POINTER(ctypes.c_uint64)
+
+
+class JobDesc(ctypes.Structure):
+    _fields_ = [
+        ("beam_energy", ctypes.c_double), ("beam_width", ctypes.c_double),
+        ("n_pixels", ctypes.c_int32), ("primaries_per_pixel", ctypes.c_int32),
+        ("pixel_center", c_double_p), ("seed", ctypes.c_uint64),
+        ("n_energy_bins", ctypes.c_int32), ("max_steps", ctypes.c_int32),
+        ("se_cutoff_ev", ctypes.c_double), ("primary_offset", ctypes.c_uint64),
+    ]
+
+
+class Tallies(ctypes.Structure):
+    _fields_ = [
+        ("back", c_u64_p), ("fwd", c_u64_p), ("se", c_u64_p),
+        ("se_sq", c_double_p), ("bse_sq", c_double_p), ("back_energy", c_double_p),
+        ("back_energy_hist", c_u64_p), ("fwd_energy_hist", c_u64_p),
+        ("elevation_hist", c_u64_p), ("azimuth_hist", c_u64_p),
+    ]
+
+
+class JobStats(ctypes.Structure):
+    _fields_ = [
+        ("primaries", ctypes.c_uint64), ("electrons", ctypes.c_uint64), ("steps", ctypes.c_uint64),
+        ("escapes", ctypes.c_uint64), ("watchdog_drops", ctypes.c_uint64), ("queue_overflow", ctypes.c_uint64),
+        ("generations", ctypes.c_int32), ("pad0", ctypes.c_int32),
+        ("device_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+    ]
+
+
+SYMBOLS = [
+    "monsel_last_error", "monsel_abi_version", "monsel_create", "monsel_destroy", "monsel_device_info",
+    "monsel_scene_footprint", "monsel_scene_upload", "monsel_scene_release", "monsel_job_footprint",
+    "monsel_job_submit", "monsel_job_poll", "monsel_job_wait", "monsel_job_read", "monsel_job_stats_get",
+    "monsel_job_release", "monsel_run", "monsel_probe_rates", "monsel_probe_elastic_angle", "monsel_probe_csd",
+    "monsel_probe_find_end_of_step", "monsel_probe_locate", "monsel_probe_nu_table",
+]
+
+
+def _fp(a):
+    return a.ctypes.data_as(c_float_p)
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed ({rc}): {last_error()}")
+
+
+class Probe:
+    """Device probes on one uploaded scene of a context."""
+
+    def __init__(self, ctx_address, scene_id):
+        self.L = gpu_lib()
+        self.ctx = ctypes.c_void_p(ctx_address)
+        self.scene = ctypes.c_uint64(scene_id)
+
+    def rates(self, msm, energies_eV, n_mech):
+        e = np.ascontiguousarray(energies_eV, dtype=np.float32)
+        out = np.zeros((e.size, n_mech + 1), np.float32)
+        _check(self.L.monsel_probe_rates(self.ctx, self.scene, int(msm), e.size, _fp(e), _fp(out)), "probe_rates")
+        return out
+
+    def elastic_angle(self, msm, element, energies_eV, u):
+        e = np.ascontiguousarray(energies_eV, dtype=np.float32)
+        uu = np.ascontiguousarray(u, dtype=np.float32).reshape(-1, 2)
+        out = np.zeros(e.size, np.float32)
+        _check(self.L.monsel_probe_elastic_angle(self.ctx, self.scene, int(msm), int(element), e.size, _fp(e), _fp(uu),
+                                                 _fp(out)), "probe_elastic_angle")
+        return out
+
+    def csd(self, msm, len_nm, energies_eV):
+        l = np.ascontiguousarray(len_nm, dtype=np.float32)
+        e = np.ascontiguousarray(energies_eV, dtype=np.float32)
+        out = np.zeros(e.size, np.float32)
+        _check(self.L.monsel_probe_csd(self.ctx, self.scene, int(msm), e.size, _fp(l), _fp(e), _fp(out)), "probe_csd")
+        return out
+
+    def find_end_of_step(self, region, p0_nm, p1_nm):
+        a = np.ascontiguousarray(p0_nm, dtype=np.float32).reshape(-1, 3)
+        b = np.ascontiguousarray(p1_nm, dtype=np.float32).reshape(-1, 3)
+        n = a.shape[0]
+        u = np.zeros(n, np.float32)
+        nxt = np.zeros(n, np.int32)
+        nrm = np.zeros((n, 3), np.float32)
+        _check(self.L.monsel_probe_find_end_of_step(self.ctx, self.scene, int(region), n, _fp(a), _fp(b), _fp(u),
+                                                    nxt.ctypes.data_as(c_int_p), _fp(nrm)), "probe_find_end_of_step")
+        return u, nxt, nrm
+
+    def locate(self, pos_nm):
+        p = np.ascontiguousarray(pos_nm, dtype=np.float32).reshape(-1, 3)
+        out = np.zeros(p.shape[0], np.int32)
+        _check(self.L.monsel_probe_locate(self.ctx, self.scene, p.shape[0], _fp(p), out.ctypes.data_as(c_int_p)),
+               "probe_locate")
+        return out
+
+    def nu_table(self, table, order, x, dim):
+        xx = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, dim)
+        out = np.zeros(xx.shape[0], np.float32)
+        _check(self.L.monsel_probe_nu_table(self.ctx, self.scene, int(table), int(order), xx.shape[0], _fp(xx), _fp(out)),
+               "probe_nu_table")
+        return out
+
+
+def probe_for(scene_obj, device=0):
+    """Upload a cudamonsel_amd.scenes.Scene and return (Probe, CompiledScene)."""
+    from . import Device
+    cs = scene_obj.monte.compileScene()
+    dev = Device.get(device)
+    sid = dev.upload(cs)
+    return Probe(dev.ctx_address(), sid), cs

@@ -1,0 +1,1053 @@
+/*
+ * transport_device.h -- FP32 device implementation of one electron step.
+ *
+ * Restates, for one GPU lane per electron, the per-step work of
+ * MonteCarloSS::takeStep (gov/nist/microanalysis/NISTMonte/MonteCarloSS.cu:107-163)
+ * and everything it calls.  Units: nm, eV; directions are unit vectors
+ * (the reference keeps theta/phi; Electron::updateDirection, Electron.cu:111-131,
+ * is the same rotation written in the electron's local frame).
+ *
+ * Geometry: every region shape is compiled into convex primitives and a
+ * truth table.  The first boundary of the composite along a ray is found by
+ * walking the primitives' entry/exit crossings in order and testing where
+ * the composite's inside state changes.  This finds the same boundary as the
+ * depth-counter walks of NormalIntersectionShape.cu:19-239 and
+ * NormalUnionShape.cu:30-271 without their recursive re-queries.
+ */
+#ifndef MONSEL_TRANSPORT_DEVICE_H
+#define MONSEL_TRANSPORT_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/monsel_gpu.h"
+#include "device_scene.h"
+
+namespace monsel_dev {
+
+#define MD __device__ __forceinline__
+
+constexpr float kPI = 3.14159265358979323846f;
+constexpr float kTwoPI = 6.28318530717958647692f;
+constexpr float kInf = __builtin_huge_valf();
+constexpr float kA0nm = 0.05291772083f;          /* Bohr radius (nm) */
+constexpr float kA0nm2 = 0.05291772083f * 0.05291772083f;
+constexpr float kMottLn50 = 3.9120230054281460f;    /* ln(50) */
+constexpr float kMottInvParam = 60.0f / 5.9914645471079810f; /* 60 / ln(400) */
+constexpr float kMaxNISTMott = 20000.0f;            /* eV */
+constexpr float kEcharge = 1.60217653e-19f;         /* J per eV */
+
+/* ----------------------------------------------------------- RNG ------- */
+
+/* Philox4x32-10 (Salmon et al., SC'11); stream layout as in the oracle. */
+MD void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
+                      uint32_t& o0, uint32_t& o1, uint32_t& o2, uint32_t& o3) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+    uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+    uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0;
+    c1 = lo1;
+    c2 = n2;
+    c3 = lo0;
+  }
+  o0 = c0;
+  o1 = c1;
+  o2 = c2;
+  o3 = c3;
+}
+
+MD uint64_t splitmix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+MD uint64_t child_lineage(uint64_t parent, uint32_t k) {
+  return splitmix64(parent ^ (0x9E3779B97F4A7C15ull * (uint64_t)(k + 1)));
+}
+
+/* electron state held in registers */
+struct Elec {
+  float px, py, pz;
+  float dx, dy, dz;
+  float E;
+  int region;
+  uint32_t primary;
+  uint32_t lin_lo, lin_hi;
+  uint32_t draws;
+  uint32_t nchild;
+  uint32_t steps;
+  uint32_t r0, r1, r2, r3;
+};
+
+MD float u01(uint32_t x) { return (float)(x >> 9) * 1.1920928955078125e-07f + 5.9604644775390625e-08f; }
+
+MD float rng_next(Elec& e, uint32_t k0, uint32_t k1) {
+  uint32_t slot = e.draws & 3u;
+  if (slot == 0) philox4x32_10(e.draws >> 2, e.lin_lo, e.lin_hi, e.primary, k0, k1, e.r0, e.r1, e.r2, e.r3);
+  uint32_t x = slot == 0 ? e.r0 : (slot == 1 ? e.r1 : (slot == 2 ? e.r2 : e.r3));
+  e.draws++;
+  return u01(x);
+}
+
+/* ----------------------------------------------------------- math ------ */
+
+/* Electron::updateDirection (Electron.cu:111-131) on a unit vector: deflect
+ * by polar angle (ca = cos, sa = sin) and azimuth (cb, sb). */
+MD void rotate_dir(float& dx, float& dy, float& dz, float ca, float sa, float cb, float sb) {
+  float st2 = dx * dx + dy * dy;
+  float cp = 1.f, sp = 0.f, st = 0.f;
+  if (st2 > 0.f) {
+    st = __builtin_sqrtf(st2);
+    float inv = 1.f / st;
+    cp = dx * inv;
+    sp = dy * inv;
+  }
+  float ct = dz;
+  float xx = cb * ct * sa + ca * st;
+  float yy = sa * sb;
+  float nx = cp * xx - sp * yy;
+  float ny = cp * yy + sp * xx;
+  float nz = ca * ct - cb * sa * st;
+  float inv = __builtin_amdgcn_rsqf(nx * nx + ny * ny + nz * nz);
+  dx = nx * inv;
+  dy = ny * inv;
+  dz = nz * inv;
+}
+
+/* direction from polar/azimuth angles (Electron(parent, theta, phi, kE)) */
+MD void dir_from_angles(float ct, float st, float cph, float sph, float& dx, float& dy, float& dz) {
+  dx = st * cph;
+  dy = st * sph;
+  dz = ct;
+}
+
+/* Lagrange cubic through unit-spaced nodes 0..3 evaluated at t (the value
+ * Neville's scheme in ULagrangeInterpolation.cu:5-33 produces). */
+MD float lagrange4(float t, float f0, float f1, float f2, float f3) {
+  float t1 = t - 1.f, t2 = t - 2.f, t3 = t - 3.f;
+  float l0 = -t1 * t2 * t3 * (1.f / 6.f);
+  float l1 = t * t2 * t3 * 0.5f;
+  float l2 = -t * t1 * t3 * 0.5f;
+  float l3 = t * t1 * t2 * (1.f / 6.f);
+  return l0 * f0 + l1 * f1 + l2 * f2 + l3 * f3;
+}
+
+MD void lagrange4_w(float t, float& l0, float& l1, float& l2, float& l3) {
+  float t1 = t - 1.f, t2 = t - 2.f, t3 = t - 3.f;
+  l0 = -t1 * t2 * t3 * (1.f / 6.f);
+  l1 = t * t2 * t3 * 0.5f;
+  l2 = -t * t1 * t3 * 0.5f;
+  l3 = t * t1 * t2 * (1.f / 6.f);
+}
+
+/* ULagrangeInterpolation::d1 index choice (ULagrangeInterpolation.cu:41-47). */
+MD int u_index0(float reduced, int len) {
+  int i0 = (int)reduced - 1;
+  i0 = i0 < 0 ? 0 : i0;
+  return i0 > len - 4 ? len - 4 : i0;
+}
+
+/* ------------------------------------------------- elastic (per element) */
+
+/* BrowningEmpiricalCrossSection.cu:24-29 (nm^2) */
+MD float browning_sigma(const DElem& el, float EeV) {
+  float e = EeV * 1.0e-3f;
+  float re = __builtin_sqrtf(e);
+  return 3.0e-4f * el.zp17 / (e + 0.005f * el.zp17 * re + 0.0007f * el.zp2 / re);
+}
+
+/* BrowningEmpiricalCrossSection.cu:31-45 -> cos(theta) */
+MD float browning_cos(const DElem& el, float EeV, float r1, float r2) {
+  float e = EeV * 1.0e-3f;
+  float r = (300.0f * e / el.z) + (el.zp3 / (3.0e5f * e));
+  if (r1 <= r / (r + 1.f)) {
+    float alpha = 7.0e-3f / e;
+    return 1.0f - ((2.0f * alpha * r2) / (alpha - r2 + 1.f));
+  }
+  return 1.f - 2.0f * r2;
+}
+
+/* ScreenedRutherfordScatteringAngle.cu:24-39 (sigma nm^2, cos theta) */
+MD float rutherford_sigma(const DElem& el, float EeV) {
+  const float K1 = 7.670843088080456e-38f / 1.60217653e-19f * 1.0e18f;
+  const float K2 = 5.44967975966321e-19f / 1.60217653e-19f;
+  float zp = el.rz13;
+  return K1 * zp * (1.f + el.z) / (EeV + K2 * zp * zp);
+}
+MD float rutherford_cos(const DElem& el, float EeV, float r) {
+  const float K = 5.44968e-19f / 1.60217653e-19f;
+  float alpha = K * el.rz23 / EeV;
+  return 1.f - 2.0f * alpha * r / (1.f + alpha - r);
+}
+
+/* NISTMottRS::totalCrossSection (NISTMottRS.cu:103-119), nm^2 */
+MD float mott_table_sigma(const DevScene& S, int mott, float EeV) {
+  float red = (__logf(EeV) - kMottLn50) * kMottInvParam;
+  int i0 = u_index0(red, kMottNE);
+  const float* f = S.mott_spwem + mott * kMottNE + i0;
+  return kA0nm2 * lagrange4(red - (float)i0, f[0], f[1], f[2], f[3]);
+}
+
+MD float elem_sigma(const DevScene& S, const DElem& el, float EeV) {
+  if (el.kind == MONSEL_MECH_BROWNING_ELASTIC) return browning_sigma(el, EeV);
+  if (el.kind == MONSEL_MECH_RUTHERFORD_ELASTIC) return rutherford_sigma(el, EeV);
+  if (EeV < el.extrap) {
+    if (el.method == 3) return el.mottAtMin * EeV / el.extrap;
+    return el.sfBrowning * browning_sigma(el, EeV);
+  }
+  if (EeV < kMaxNISTMott) return mott_table_sigma(S, el.mott, EeV);
+  return rutherford_sigma(el, EeV);
+}
+
+/* NISTMottRS::randomScatteringAngle table branch: bicubic Lagrange on X1
+ * over (ln E, U) (NISTMottRS.cu:134-150, ULagrangeInterpolation.cu:55-78);
+ * returns cos(theta) clamped as the reference does. */
+MD float mott_cos(const DevScene& S, int mott, float EeV, float u) {
+  float rx = (__logf(EeV) - kMottLn50) * kMottInvParam;
+  int i0 = u_index0(rx, kMottNE);
+  float ry = u * 200.0f;
+  int j0 = u_index0(ry, kMottNQ);
+  float wx0, wx1, wx2, wx3, wy0, wy1, wy2, wy3;
+  lagrange4_w(rx - (float)i0, wx0, wx1, wx2, wx3);
+  lagrange4_w(ry - (float)j0, wy0, wy1, wy2, wy3);
+  const float* base = S.mott_x1 + (size_t)mott * (kMottNE * kMottNQ) + i0 * kMottNQ + j0;
+  float q = 0.f;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const float* row = base + i * kMottNQ;
+    float v = wy0 * row[0] + wy1 * row[1] + wy2 * row[2] + wy3 * row[3];
+    float wx = i == 0 ? wx0 : (i == 1 ? wx1 : (i == 2 ? wx2 : wx3));
+    q += wx * v;
+  }
+  float com = 1.0f - 2.0f * q * q;
+  return com > -1.0f ? (com < 1.0f ? com : 1.0f) : -1.0f;
+}
+
+/* ------------------------------------------------------------- CSD ----- */
+
+/* JoyLuoNieminenCSD::compute (JoyLuoNieminenCSD.cu:91-121) for one segment;
+ * returns false when the Joy-Luo loss exceeds 10% and the segment must be
+ * halved. */
+MD bool csd_segment(const DCSD& c, float len, float kE, float& dE) {
+  dE = 0.f;
+  if (c.nce == 0 || kE < c.minE || kE <= 0.f) return true;
+  if (kE <= c.breakE) {
+    /* kE/(1 + 1.5 gamma len kE^1.5)^(2/3) - kE, written with log1p/expm1 */
+    float eps = 1.5f * c.gamma * len * kE * __builtin_sqrtf(kE);
+    dE = kE * expm1f(-(2.f / 3.f) * log1pf(eps));
+    return true;
+  }
+  float loss = 0.f;
+  for (int i = 0; i < c.nce; ++i) loss += c.coef[i] * __logf(c.recipJ[i] * kE + c.beta[i]);
+  loss *= len / kE;
+  if (loss <= 0.1f * kE) {
+    dE = -loss;
+    return true;
+  }
+  return false;
+}
+
+/* The recursive halving of JoyLuoNieminenCSD.cu:154-159 as an explicit
+ * depth-first walk: bit d of `pending` marks a second half at depth d. */
+MD float csd_compute(const DCSD& c, float len, float kE) {
+  if (!c.active) return 0.f;
+  float E = kE, total = 0.f;
+  uint32_t pending = 0;
+  int d = 0;
+  for (int it = 0; it < 128; ++it) {
+    float seg = ldexpf(len, -d);
+    float dE;
+    if (!csd_segment(c, seg, E, dE) && d < 30) {
+      pending |= 1u << (d + 1);
+      d += 1;
+      continue;
+    }
+    if (!csd_segment(c, seg, E, dE)) {
+      float loss = 0.f;
+      for (int i = 0; i < c.nce; ++i) loss += c.coef[i] * __logf(c.recipJ[i] * E + c.beta[i]);
+      dE = -loss * seg / E;
+    }
+    E += dE;
+    total += dE;
+    if (pending == 0) break;
+    d = 31 - __clz(pending);
+    pending &= ~(1u << d);
+  }
+  return total;
+}
+
+/* ------------------------------------------------------- NU tables ----- */
+
+/* NULagrangeInterpolation::locate (NULagrangeInterpolation.cu:5-68) on an
+ * ascending or descending grid: first index of the order+1 window. */
+MD int nu_locate(const float* xs, int n, int order, float x, int& nearest) {
+  int lowlim = -1, uplim = n, maxindex = n - 1;
+  bool asc = xs[maxindex] > xs[0];
+  while (uplim - lowlim > 1) {
+    int mid = (uplim + lowlim) >> 1;
+    if ((x > xs[mid]) == asc) lowlim = mid;
+    else uplim = mid;
+  }
+  if (lowlim < 0) {
+    nearest = 0;
+    return 0;
+  }
+  if (uplim > maxindex) {
+    nearest = maxindex;
+    return maxindex - order;
+  }
+  float xl = xs[lowlim], xu = xs[uplim];
+  if (asc) nearest = (x - xl) <= (xu - x) ? lowlim : uplim;
+  else nearest = (xl - x) <= (x - xu) ? lowlim : uplim;
+  int first = lowlim, last = uplim;
+  while ((last - first) < order && first > 0 && last < maxindex) {
+    if ((((xl - xs[first - 1]) + xu) - xs[last + 1] >= 0.f) == asc) last += 1;
+    else first -= 1;
+  }
+  if (last >= maxindex) return maxindex - order;
+  if (first <= 0) return 0;
+  return first;
+}
+
+/* Lagrange polynomial through (xs[k], f[k]), k = 0..order (order <= 3). */
+MD float nu_poly(const float* xs, const float* f, int order, float x) {
+  float y = 0.f;
+  for (int i = 0; i <= order; ++i) {
+    float w = 1.f;
+    for (int j = 0; j <= order; ++j)
+      if (j != i) w *= (x - xs[j]) / (xs[i] - xs[j]);
+    y += w * f[i];
+  }
+  return y;
+}
+
+MD float nu_d1(const float* f, const float* xs, int n, int order, float x) {
+  int nearest;
+  int i0 = nu_locate(xs, n, order, x, nearest);
+  return nu_poly(xs + i0, f + i0, order, x);
+}
+
+/* NULagrangeInterpolation::d2 (:169-192); f is an [na][nb] block. */
+MD float nu_d2(const float* f, const float* xa, int na, const float* xb, int nb, int order, float x0, float x1) {
+  int nearest;
+  int i0 = nu_locate(xa, na, order, x0, nearest);
+  int j0 = nu_locate(xb, nb, order, x1, nearest);
+  float y[4];
+  for (int i = 0; i <= order; ++i) y[i] = nu_poly(xb + j0, f + (size_t)(i0 + i) * nb + j0, order, x1);
+  return nu_poly(xa + i0, y, order, x0);
+}
+
+/* NULagrangeInterpolation::d3 (:215-240); compat_d3 reproduces line 236. */
+MD float nu_d3(const DevScene& S, const DNUTable& t, int order, float x0, float x1, float x2, bool compat_d3) {
+  const float* xa = S.nu_data + t.xoff[0];
+  const float* xb = S.nu_data + t.xoff[1];
+  const float* xc = S.nu_data + t.xoff[2];
+  const float* v = S.nu_data + t.voff;
+  int nearest;
+  int i0 = nu_locate(xa, t.n[0], order, x0, nearest);
+  int j0 = nu_locate(xb, t.n[1], order, x1, nearest);
+  int k0 = nu_locate(xc, t.n[2], order, x2, nearest);
+  float y[4];
+  for (int i = 0; i <= order; ++i) {
+    float z[4];
+    const float* blk = v + (size_t)(i0 + i) * t.n[1] * t.n[2];
+    for (int j = 0; j <= order; ++j) z[j] = nu_poly(xc + k0, blk + (size_t)(j0 + j) * t.n[2] + k0, order, x2);
+    y[i] = nu_poly(xb + j0, z, order, x1);
+  }
+  return nu_poly(compat_d3 ? xa : xa + i0, y, order, x0);
+}
+
+MD float nu_d4(const DevScene& S, const DNUTable& t, int order, const float* x, bool compat_d3) {
+  const float* xa = S.nu_data + t.xoff[0];
+  int nearest;
+  int i0 = nu_locate(xa, t.n[0], order, x[0], nearest);
+  float y[4];
+  for (int i = 0; i <= order; ++i) {
+    DNUTable sub;
+    sub.dim = 3;
+    for (int k = 0; k < 3; ++k) {
+      sub.n[k] = t.n[k + 1];
+      sub.xoff[k] = t.xoff[k + 1];
+    }
+    sub.voff = t.voff + (i0 + i) * t.n[1] * t.n[2] * t.n[3];
+    y[i] = nu_d3(S, sub, order, x[1], x[2], x[3], compat_d3);
+  }
+  return nu_poly(xa + i0, y, order, x[0]);
+}
+
+MD float nu_interp(const DevScene& S, int table, int order, const float* x, bool compat_d3) {
+  const DNUTable& t = S.nu[table];
+  const float* v = S.nu_data + t.voff;
+  switch (t.dim) {
+    case 1: return nu_d1(v, S.nu_data + t.xoff[0], t.n[0], order, x[0]);
+    case 2: return nu_d2(v, S.nu_data + t.xoff[0], t.n[0], S.nu_data + t.xoff[1], t.n[1], order, x[0], x[1]);
+    case 3: return nu_d3(S, t, order, x[0], x[1], x[2], compat_d3);
+    case 4: return nu_d4(S, t, order, x, compat_d3);
+  }
+  return 0.f;
+}
+
+/* ------------------------------------------------------ mechanisms ----- */
+
+MD float elastic_rate(const DevScene& S, const DMech& m, float EeV) {
+  float tot = 0.f;
+  for (int i = 0; i < m.nelem; ++i) {
+    const DElem& el = S.elems[m.elem0 + i];
+    tot += elem_sigma(S, el, EeV) * el.natoms;
+  }
+  return tot;
+}
+
+/* ScatterMechanism::scatterRate (1/nm) */
+MD float mech_rate(const DevScene& S, const DMech& m, float EeV) {
+  switch (m.kind) {
+    case MONSEL_MECH_NISTMOTT_ELASTIC:
+    case MONSEL_MECH_BROWNING_ELASTIC:
+    case MONSEL_MECH_RUTHERFORD_ELASTIC:
+      return elastic_rate(S, m, EeV);
+    case MONSEL_MECH_FITTED_INELASTIC: /* FittedInelSM.cu:20-25 */
+      if (EeV <= m.p[0]) return 0.f;
+      return -csd_compute(m.csd, 0.1f, EeV) * 10.f * m.p[2];
+    case MONSEL_MECH_POLARON_TRAP: /* GanachaudMokraniPolaronTrapSM.cu:21-27 */
+      if (EeV > m.p[2]) return 0.f;
+      return m.p[0] * __expf(-m.p[1] * EeV);
+    case MONSEL_MECH_PHONON: { /* GanachaudMokraniPhononInelasticSM.cu:46-72 */
+      if (EeV < m.p[0]) return 0.f;
+      float x = m.p[0] / EeV;
+      if (x < 0.1f) return m.p[1] * x * __logf(4.f / x);
+      if (x >= 1.f) return 0.f;
+      float temp = __builtin_sqrtf(1.f - x);
+      return m.p[1] * x * __logf((1.f + temp) / (1.f - temp));
+    }
+    case MONSEL_MECH_TABULATED_INELASTIC: { /* TabulatedInelasticSM.cu:436-459 */
+      float kE = (EeV + m.p[TP_OFFSET]) * kEcharge;
+      if (kE < m.p[TP_IIMFPDOM0]) return 0.f;
+      return m.p[TP_RATEMULT] * nu_interp(S, m.table[0], 1, &kE, false) * 1.0e-9f;
+    }
+  }
+  return 0.f;
+}
+
+/* secondary produced by a scatter event (new Electron(*pe, theta, phi, E)
+ * followed by an optional relative rotation) */
+struct Spawn {
+  bool has;
+  float dx, dy, dz;
+  float E;
+};
+
+/* TabulatedInelasticSM.cu:103-151 (energies in J) */
+MD float e0_from_dispersion(float Eq, float deltaE) {
+  if (Eq == 0.f) return deltaE;
+  float x = Eq / deltaE;
+  float y = 3.77300614251479e17f * Eq;
+  float x13 = cbrtf(x);
+  float x2 = x * x;
+  float c1 = x2 * x2 * (27.f + (18.f * y) + (2.f * y * y));
+  float c2 = x2 * (27.f + (4.f * y));
+  float c3 = c2 - 27.f;
+  float c4 = x2 * (3.f + y);
+  float c6 = 1.f - x2;
+  if (c3 > 0.f) {
+    float tanPart = atan2f(3.f * c6 * sqrtf(3.f * c3 * c6), (18.f * c4) - c1 - 27.f);
+    float trigPart = cosf(tanPart / 3.f);
+    float prefactor = 2.f * x * sqrtf(y * (y - (c6 * (y + 6.f))));
+    trigPart *= prefactor;
+    return deltaE * sqrtf(((3.f - c4) + trigPart) / 3.f);
+  }
+  float c5 = cbrtf(-c1 + (18.f * c4) + (3.f * (-9.f + (c6 * sqrtf(-(3.f * c3 * c6))))));
+  float term1 = -2.f * c4;
+  float x23 = x13 * x13;
+  float x43 = x * x13;
+  float y13 = cbrtf(y);
+  float y23 = y13 * y13;
+  float x43y23 = x43 * y23;
+  float two13 = 1.2599210498948732f;
+  float term2 = -12.f * two13 * x43y23 * c6;
+  float term3 = 2.f * two13 * x43y23 * x2 * y;
+  float term23 = (term2 + term3) / c5;
+  float term4 = two13 * two13 * x23 * y13 * c5;
+  return deltaE * sqrtf((6.f + term1 + term23 + term4) / 6.f);
+}
+
+/* TabulatedInelasticSM::pickBE (:153-207), default branching ratios; J. */
+MD float pick_be(const DMech& m, float Eq, float deltaE) {
+  int ncore = (int)m.p[TP_NCORE];
+  if (ncore > 0 && deltaE <= m.p[TP_CORE0]) return 0.f;
+  float eIon = m.p[TP_E0DISP] != 0.f ? e0_from_dispersion(Eq, deltaE) : deltaE;
+  int i = 0;
+  while (i < ncore && m.p[TP_CORE0 + i] <= eIon) ++i;
+  return i == 0 ? 0.f : m.p[TP_CORE0 + i - 1];
+}
+
+MD void isotropic_dir(float u1, float u2, float& dx, float& dy, float& dz) {
+  /* theta = acos(1 - 2 u1), phi = 2 pi u2 */
+  float ct = 1.f - 2.f * u1;
+  float st = __builtin_sqrtf(fmaxf(0.f, 1.f - ct * ct));
+  float sp, cp;
+  __sincosf(kTwoPI * u2, &sp, &cp);
+  dx = st * cp;
+  dy = st * sp;
+  dz = ct;
+}
+
+/* TabulatedInelasticSM::scatter (TabulatedInelasticSM.cu:209-434).  Table
+ * lookups in SI units (J), results converted back to eV. */
+MD Spawn tabulated_scatter(const DevScene& S, const DMech& m, Elec& e, uint32_t k0, uint32_t k1) {
+  Spawn se;
+  se.has = false;
+  const bool cd3 = (S.compat & MONSEL_COMPAT_D3_ABSCISSAE) != 0;
+  const float J = kEcharge;
+  float kE0 = e.E * J;
+  float kE = kE0 + m.p[TP_OFFSET] * J;
+  if (kE < m.p[TP_EIDOM0]) return se;
+  float rnd0 = rng_next(e, k0, k1), rnd1 = rng_next(e, k0, k1), rnd2 = rng_next(e, k0, k1), rnd3 = rng_next(e, k0, k1);
+  float in[3] = {kE, rnd0, 0.f};
+  float deltaE = kE * nu_interp(S, m.table[1], 3, in, cd3);
+  float gap = m.p[TP_GAP] * J, bandgap = m.p[TP_BANDGAP] * J;
+  if (deltaE < gap && deltaE > 0.95f * gap) deltaE = gap;
+  float d0x = e.dx, d0y = e.dy, d0z = e.dz;
+  float theta = 0.f, sphi = 0.f, cphi = 1.f, phi = 0.f;
+  float offsetFermi = m.p[TP_OFFSETFERMI] * J;
+  if (deltaE >= bandgap) {
+    in[1] = (deltaE - gap) / (kE - offsetFermi - 2.f * gap);
+    in[1] = in[1] > 1.f ? 1.f : (in[1] < 0.f ? 0.f : in[1]);
+    in[2] = rnd1;
+    theta = nu_interp(S, m.table[2], 3, in, cd3);
+    phi = kTwoPI * rnd2;
+    __sincosf(phi, &sphi, &cphi);
+    float st, ct;
+    __sincosf(theta, &st, &ct);
+    rotate_dir(e.dx, e.dy, e.dz, ct, st, cphi, sphi);
+  }
+  e.E = (kE0 - deltaE) / J;
+  if (deltaE < bandgap) return se;
+  float Eq = (2.f * kE) - deltaE - (2.f * sqrtf(kE * (kE - deltaE)) * cosf(theta));
+  float cb = m.p[TP_CB] * J, minEgen = m.p[TP_MINEGEN] * J, bEref = m.p[TP_BEREF] * J;
+  float offset = m.p[TP_OFFSET] * J;
+  float energySE;
+  int method = m.method;
+  /* SE direction relative to the PE's original direction: thetaSE = pi/2 - theta,
+   * phiSE = phi + pi */
+  auto rel_dir = [&](float& sx, float& sy, float& sz) {
+    sx = d0x;
+    sy = d0y;
+    sz = d0z;
+    float st, ct;
+    __sincosf(0.5f * kPI - theta, &st, &ct);
+    rotate_dir(sx, sy, sz, ct, st, -cphi, -sphi);
+  };
+  auto se0 = [&](float dE) {
+    float in2[2] = {dE, rnd3};
+    float e0 = nu_interp(S, m.table[3], 3, in2, cd3);
+    if (e0 < m.p[TP_SE0RANGE0] * J) e0 = m.p[TP_SE0RANGE0] * J;
+    else if (e0 > m.p[TP_SE0RANGE1] * J) e0 = m.p[TP_SE0RANGE1] * J;
+    return e0;
+  };
+  if (method == 1) {
+    energySE = (deltaE + bEref) - pick_be(m, Eq, deltaE);
+    if (energySE + cb < minEgen) return se;
+    rel_dir(se.dx, se.dy, se.dz);
+  } else if (method == 2) {
+    float be = pick_be(m, Eq, deltaE);
+    if (be > 0.f) energySE = (deltaE + bEref) - be;
+    else energySE = (deltaE + se0(deltaE)) - offset;
+    if (energySE + cb < minEgen) return se;
+    rel_dir(se.dx, se.dy, se.dz);
+  } else {
+    float be = pick_be(m, Eq, deltaE);
+    if (be > 0.f) {
+      energySE = (deltaE + bEref) - be;
+      if (energySE + cb < minEgen) return se;
+      float u1 = rng_next(e, k0, k1), u2 = rng_next(e, k0, k1);
+      isotropic_dir(u1, u2, se.dx, se.dy, se.dz);
+    } else {
+      float root = 2.f * sqrtf(offsetFermi * (offsetFermi + deltaE));
+      float sum = (2.f * offsetFermi) + deltaE;
+      if ((sum - root) <= Eq && Eq <= (sum + root)) {
+        /* simESEf (TabulatedInelasticSM.cu:87-101) */
+        float q = sqrtf(Eq);
+        float kz = (deltaE - Eq) / 2.f / q;
+        float kzf = kz + q;
+        float Ezq = kzf * kzf;
+        float minE = (offsetFermi + bandgap) - Ezq;
+        if (minE < 0.f) minE = 0.f;
+        float maxE = offsetFermi - (kz * kz);
+        float Exy = (minE * (1.f - rnd3)) + (maxE * rnd3);
+        float ESEf = Exy + Ezq;
+        float thSE = acosf(kzf / sqrtf(ESEf));
+        energySE = ESEf - offset;
+        if (energySE + cb < minEgen) return se;
+        /* TabulatedInelasticSM.cu:388-395: the q-vector direction (pi/2 - theta,
+         * phi + pi) is deflected by (thSE, 2 pi U) in the PE frame, and the SE
+         * is the PE's original direction rotated by the resulting angles. */
+        float st, ct;
+        __sincosf(0.5f * kPI - theta, &st, &ct);
+        float wx, wy, wz;
+        dir_from_angles(ct, st, -cphi, -sphi, wx, wy, wz);
+        float sp2, cp2, st2, ct2;
+        __sincosf(kTwoPI * rng_next(e, k0, k1), &sp2, &cp2);
+        __sincosf(thSE, &st2, &ct2);
+        rotate_dir(wx, wy, wz, ct2, st2, cp2, sp2);
+        float wst = __builtin_sqrtf(fmaxf(0.f, wx * wx + wy * wy));
+        float wcp = 1.f, wsp = 0.f;
+        if (wst > 0.f) {
+          wcp = wx / wst;
+          wsp = wy / wst;
+        }
+        se.dx = d0x;
+        se.dy = d0y;
+        se.dz = d0z;
+        rotate_dir(se.dx, se.dy, se.dz, wz, wst, wcp, wsp);
+      } else {
+        energySE = (deltaE + se0(deltaE)) - offset;
+        if (energySE + cb < minEgen) return se;
+        float u1 = rng_next(e, k0, k1), u2 = rng_next(e, k0, k1);
+        isotropic_dir(u1, u2, se.dx, se.dy, se.dz);
+      }
+    }
+  }
+  se.has = true;
+  se.E = energySE / J;
+  return se;
+}
+
+/* ScatterMechanism::scatter */
+MD Spawn mech_scatter(const DevScene& S, const DMech& m, Elec& e, float prevE, bool& trapped, uint32_t k0,
+                      uint32_t k1) {
+  Spawn se;
+  se.has = false;
+  switch (m.kind) {
+    case MONSEL_MECH_NISTMOTT_ELASTIC:
+    case MONSEL_MECH_BROWNING_ELASTIC:
+    case MONSEL_MECH_RUTHERFORD_ELASTIC: { /* SelectableElasticSM.cu:49-67 */
+      float cum[kMaxElem];
+      float tot = 0.f;
+      for (int i = 0; i < m.nelem; ++i) {
+        const DElem& el = S.elems[m.elem0 + i];
+        tot += elem_sigma(S, el, prevE) * el.natoms;
+        cum[i] = tot;
+      }
+      float r = rng_next(e, k0, k1) * tot;
+      int idx = 0;
+      while (idx < m.nelem - 1 && cum[idx] < r) ++idx;
+      const DElem& el = S.elems[m.elem0 + idx];
+      float ca;
+      if (el.kind == MONSEL_MECH_BROWNING_ELASTIC ||
+          (el.kind == MONSEL_MECH_NISTMOTT_ELASTIC && prevE < el.extrap)) {
+        float r1 = rng_next(e, k0, k1), r2 = rng_next(e, k0, k1);
+        ca = browning_cos(el, prevE, r1, r2);
+      } else if (el.kind == MONSEL_MECH_RUTHERFORD_ELASTIC || prevE >= kMaxNISTMott) {
+        ca = rutherford_cos(el, prevE, rng_next(e, k0, k1));
+      } else {
+        ca = mott_cos(S, el.mott, prevE, rng_next(e, k0, k1));
+      }
+      ca = fminf(1.f, fmaxf(-1.f, ca));
+      float sa = __builtin_sqrtf(fmaxf(0.f, 1.f - ca * ca));
+      float sb, cb;
+      __sincosf(kTwoPI * rng_next(e, k0, k1), &sb, &cb);
+      rotate_dir(e.dx, e.dy, e.dz, ca, sa, cb, sb);
+      return se;
+    }
+    case MONSEL_MECH_FITTED_INELASTIC: { /* FittedInelSM.cu:13-18 */
+      float u_phi = rng_next(e, k0, k1);
+      float u_th = rng_next(e, k0, k1);
+      isotropic_dir(u_th, u_phi, se.dx, se.dy, se.dz);
+      se.E = m.p[1];
+      se.has = true;
+      return se;
+    }
+    case MONSEL_MECH_POLARON_TRAP: /* GanachaudMokraniPolaronTrapSM.cu:15-20 */
+      e.E = 0.f;
+      trapped = true;
+      return se;
+    case MONSEL_MECH_PHONON: { /* GanachaudMokraniPhononInelasticSM.cu:19-44 */
+      float kE0 = e.E;
+      if (kE0 < m.p[0]) return se;
+      float x = m.p[0] / kE0;
+      float r0 = rng_next(e, k0, k1), r1 = rng_next(e, k0, k1);
+      float ct;
+      if (x < 0.1f) ct = 1.f + ((x * x) - (exp2f(4.f * r0) * powf(x, 2.f - 2.f * r0))) / 8.f;
+      else {
+        float root = __builtin_sqrtf(1.f - x);
+        float temp = powf(((-2.f * (1.f + root)) + x) / ((-2.f * (1.f - root)) + x), r0);
+        ct = temp + (((x - 2.f) * (temp - 1.f)) / 2.f / root);
+      }
+      ct = fminf(1.f, fmaxf(-1.f, ct));
+      float st = __builtin_sqrtf(fmaxf(0.f, 1.f - ct * ct));
+      float sb, cb;
+      __sincosf(kTwoPI * r1, &sb, &cb);
+      rotate_dir(e.dx, e.dy, e.dz, ct, st, cb, sb);
+      e.E = kE0 - m.p[0];
+      return se;
+    }
+    case MONSEL_MECH_TABULATED_INELASTIC:
+      return tabulated_scatter(S, m, e, k0, k1);
+  }
+  return se;
+}
+
+/* --------------------------------------------------------- geometry ---- */
+
+struct Interval {
+  float tin, tout;
+  int fin, fout; /* face codes: plane index, or -1 cap0 / -2 cap1 / -3 side */
+};
+
+MD bool plane_tie_break(float nx, float ny, float nz) {
+  /* NormalMultiPlaneShape.cu:26-38 */
+  if (nx < 0.f) return false;
+  if (nx == 0.f) {
+    if (ny < 0.f) return false;
+    if (ny == 0.f && nz < 0.f) return false;
+  }
+  return true;
+}
+
+/* Parameter interval of the line p + t d inside a convex primitive. */
+MD Interval prim_interval(const DevScene& S, const DPrim& pr, float px, float py, float pz, float dx, float dy,
+                          float dz) {
+  Interval iv;
+  iv.tin = -kInf;
+  iv.tout = kInf;
+  iv.fin = iv.fout = -4;
+  if (pr.kind == PRIM_PLANES) {
+    for (int k = 0; k < pr.n_planes; ++k) {
+      float4 pl = S.planes[pr.first_plane + k];
+      float num = pl.x * px + pl.y * py + pl.z * pz - pl.w;
+      float den = pl.x * dx + pl.y * dy + pl.z * dz;
+      if (den == 0.f) {
+        if (num < 0.f || (num == 0.f && plane_tie_break(pl.x, pl.y, pl.z))) continue;
+        iv.tin = kInf;
+        iv.tout = -kInf;
+        return iv;
+      }
+      float t = -num / den;
+      if (den > 0.f) {
+        if (t < iv.tout) {
+          iv.tout = t;
+          iv.fout = k;
+        }
+      } else if (t > iv.tin) {
+        iv.tin = t;
+        iv.fin = k;
+      }
+    }
+    return iv;
+  }
+  if (pr.kind == PRIM_CYLINDER) {
+    float wx = px - pr.e0[0], wy = py - pr.e0[1], wz = pz - pr.e0[2];
+    float apar = wx * pr.ax[0] + wy * pr.ax[1] + wz * pr.ax[2];
+    float dpar = dx * pr.ax[0] + dy * pr.ax[1] + dz * pr.ax[2];
+    float sin_ = -kInf, sout = kInf;
+    int fsin = -4, fsout = -4;
+    if (dpar != 0.f) {
+      float inv = 1.f / dpar;
+      float t0 = -apar * inv, t1 = (pr.len - apar) * inv;
+      if (dpar > 0.f) {
+        sin_ = t0; fsin = -1; sout = t1; fsout = -2;
+      } else {
+        sin_ = t1; fsin = -2; sout = t0; fsout = -1;
+      }
+    } else if (!(apar >= 0.f && apar <= pr.len)) {
+      iv.tin = kInf;
+      iv.tout = -kInf;
+      return iv;
+    }
+    /* radial part from the components perpendicular to the axis */
+    float qx = wx - apar * pr.ax[0], qy = wy - apar * pr.ax[1], qz = wz - apar * pr.ax[2];
+    float vx = dx - dpar * pr.ax[0], vy = dy - dpar * pr.ax[1], vz = dz - dpar * pr.ax[2];
+    float A = vx * vx + vy * vy + vz * vz;
+    float B = qx * vx + qy * vy + qz * vz;
+    float C = qx * qx + qy * qy + qz * qz - pr.r2;
+    float rin = -kInf, rout = kInf;
+    if (A > 1e-12f) {
+      float disc = B * B - A * C;
+      if (disc < 0.f) {
+        iv.tin = kInf;
+        iv.tout = -kInf;
+        return iv;
+      }
+      float sq = __builtin_sqrtf(disc);
+      float q = -(B + (B >= 0.f ? sq : -sq));
+      float ra = q / A, rb = (q != 0.f) ? C / q : ra;
+      rin = fminf(ra, rb);
+      rout = fmaxf(ra, rb);
+    } else if (C > 0.f) {
+      iv.tin = kInf;
+      iv.tout = -kInf;
+      return iv;
+    }
+    if (sin_ >= rin) { iv.tin = sin_; iv.fin = fsin; }
+    else { iv.tin = rin; iv.fin = -3; }
+    if (sout <= rout) { iv.tout = sout; iv.fout = fsout; }
+    else { iv.tout = rout; iv.fout = -3; }
+    return iv;
+  }
+  /* sphere */
+  float cx = px - pr.e0[0], cy = py - pr.e0[1], cz = pz - pr.e0[2];
+  float B = cx * dx + cy * dy + cz * dz;
+  float C = cx * cx + cy * cy + cz * cz - pr.r2;
+  float disc = B * B - C;
+  if (disc < 0.f) {
+    iv.tin = kInf;
+    iv.tout = -kInf;
+    return iv;
+  }
+  float sq = __builtin_sqrtf(disc);
+  iv.tin = -B - sq;
+  iv.tout = -B + sq;
+  iv.fin = iv.fout = -3;
+  return iv;
+}
+
+/* Outward normal of primitive face `f` at point q. */
+MD void prim_normal(const DevScene& S, const DPrim& pr, int f, float qx, float qy, float qz, float& nx, float& ny,
+                    float& nz) {
+  if (pr.kind == PRIM_PLANES) {
+    float4 pl = S.planes[pr.first_plane + (f < 0 ? 0 : f)];
+    nx = pl.x;
+    ny = pl.y;
+    nz = pl.z;
+    return;
+  }
+  if (pr.kind == PRIM_CYLINDER) {
+    if (f == -1) { nx = -pr.ax[0]; ny = -pr.ax[1]; nz = -pr.ax[2]; return; }
+    if (f == -2) { nx = pr.ax[0]; ny = pr.ax[1]; nz = pr.ax[2]; return; }
+    float wx = qx - pr.e0[0], wy = qy - pr.e0[1], wz = qz - pr.e0[2];
+    float apar = wx * pr.ax[0] + wy * pr.ax[1] + wz * pr.ax[2];
+    wx -= apar * pr.ax[0];
+    wy -= apar * pr.ax[1];
+    wz -= apar * pr.ax[2];
+    float inv = __builtin_amdgcn_rsqf(wx * wx + wy * wy + wz * wz);
+    nx = wx * inv;
+    ny = wy * inv;
+    nz = wz * inv;
+    return;
+  }
+  float wx = qx - pr.e0[0], wy = qy - pr.e0[1], wz = qz - pr.e0[2];
+  float inv = __builtin_amdgcn_rsqf(wx * wx + wy * wy + wz * wz);
+  nx = wx * inv;
+  ny = wy * inv;
+  nz = wz * inv;
+}
+
+MD bool prim_contains(const DevScene& S, const DPrim& pr, float px, float py, float pz) {
+  if (pr.kind == PRIM_PLANES) {
+    /* MultiPlaneShape.cu:23-27 for every plane */
+    for (int k = 0; k < pr.n_planes; ++k) {
+      float4 pl = S.planes[pr.first_plane + k];
+      if (!(pl.x * px + pl.y * py + pl.z * pz - pl.w <= 0.f)) return false;
+    }
+    return true;
+  }
+  float wx = px - pr.e0[0], wy = py - pr.e0[1], wz = pz - pr.e0[2];
+  if (pr.kind == PRIM_CYLINDER) {
+    /* CylindricalShape.cu:41-45 */
+    float u = (pr.axv[0] * wx + pr.axv[1] * wy + pr.axv[2] * wz) / pr.len2;
+    float rx = wx - u * pr.axv[0], ry = wy - u * pr.axv[1], rz = wz - u * pr.axv[2];
+    return u >= 0.f && u <= 1.f && (rx * rx + ry * ry + rz * rz) <= pr.r2;
+  }
+  return wx * wx + wy * wy + wz * wz <= pr.r2;
+}
+
+MD bool truth(const DShape& sh, uint32_t bits) { return (sh.truth[bits >> 5] >> (bits & 31u)) & 1u; }
+
+MD bool shape_contains(const DevScene& S, int s, float px, float py, float pz) {
+  const DShape& sh = S.shapes[s];
+  uint32_t bits = 0;
+  for (int k = 0; k < sh.n_prims; ++k)
+    if (prim_contains(S, S.prims[sh.first_prim + k], px, py, pz)) bits |= 1u << k;
+  return truth(sh, bits);
+}
+
+struct ShapeHit {
+  float t;      /* kInf if none within tmax */
+  float nx, ny, nz;
+};
+
+/* First boundary of the composite shape along p + t d, 0 < t <= tmax. */
+MD ShapeHit shape_first(const DevScene& S, int s, float px, float py, float pz, float dx, float dy, float dz,
+                        float tmax) {
+  const DShape& sh = S.shapes[s];
+  ShapeHit h;
+  h.t = kInf;
+  h.nx = h.ny = h.nz = 0.f;
+  float tin[kMaxPrims], tout[kMaxPrims];
+  int fin[kMaxPrims], fout[kMaxPrims];
+  uint32_t bits = 0;
+  const int n = sh.n_prims;
+#pragma unroll
+  for (int k = 0; k < kMaxPrims; ++k) {
+    tin[k] = -kInf;
+    tout[k] = -kInf;
+    fin[k] = fout[k] = -4;
+    if (k < n) {
+      Interval iv = prim_interval(S, S.prims[sh.first_prim + k], px, py, pz, dx, dy, dz);
+      if (!(iv.tin < iv.tout)) {
+        iv.tin = -kInf;
+        iv.tout = -kInf;
+      }
+      tin[k] = iv.tin;
+      tout[k] = iv.tout;
+      fin[k] = iv.fin;
+      fout[k] = iv.fout;
+      if (iv.tin <= 0.f && iv.tout > 0.f) bits |= 1u << k;
+    }
+  }
+  bool state = truth(sh, bits);
+  float tcur = 0.f;
+  for (int it = 0; it < 2 * kMaxPrims; ++it) {
+    float tnext = kInf;
+#pragma unroll
+    for (int k = 0; k < kMaxPrims; ++k) {
+      float nk = tin[k] > tcur ? tin[k] : (tout[k] > tcur ? tout[k] : kInf);
+      tnext = fminf(tnext, nk);
+    }
+    if (!(tnext <= tmax)) return h;
+    int first = -1, firstFace = -4, second = -1, secondFace = -4;
+#pragma unroll
+    for (int k = 0; k < kMaxPrims; ++k) {
+      bool isIn = tin[k] == tnext && tin[k] > tcur;
+      bool isOut = !isIn && tout[k] == tnext && tout[k] > tcur;
+      if (isIn || isOut) {
+        bits ^= 1u << k;
+        int f = isIn ? fin[k] : fout[k];
+        if (first < 0) { first = k; firstFace = f; }
+        else if (second < 0) { second = k; secondFace = f; }
+      }
+    }
+    bool ns = truth(sh, bits);
+    if (ns != state) {
+      float qx = px + tnext * dx, qy = py + tnext * dy, qz = pz + tnext * dz;
+      float nx, ny, nz;
+      prim_normal(S, S.prims[sh.first_prim + first], firstFace, qx, qy, qz, nx, ny, nz);
+      if (second >= 0) {
+        /* simultaneous crossings: mean of the two normals
+         * (NormalIntersectionShape.cu:229-236) */
+        float mx, my, mz;
+        prim_normal(S, S.prims[sh.first_prim + second], secondFace, qx, qy, qz, mx, my, mz);
+        nx = 0.5f * (nx + mx);
+        ny = 0.5f * (ny + my);
+        nz = 0.5f * (nz + mz);
+      }
+      h.t = tnext;
+      h.nx = nx;
+      h.ny = ny;
+      h.nz = nz;
+      return h;
+    }
+    state = ns;
+    tcur = tnext;
+  }
+  return h;
+}
+
+/* RegionBase::containingSubRegion (RegionBase.cu:50-60) from region r. */
+MD int containing_sub_region(const DevScene& S, int r, float px, float py, float pz) {
+  if (!shape_contains(S, S.regions[r].shape, px, py, pz)) return -1;
+  for (int guard = 0; guard < 64; ++guard) {
+    const DRegion& rg = S.regions[r];
+    int next = -1;
+    for (int c = 0; c < rg.n_children; ++c) {
+      int ch = S.children[rg.first_child + c];
+      if (shape_contains(S, S.regions[ch].shape, px, py, pz)) {
+        next = ch;
+        break;
+      }
+    }
+    if (next < 0) return r;
+    r = next;
+  }
+  return r;
+}
+
+MD bool is_containing_region(const DevScene& S, int r, int target) {
+  int a = S.regions[target].parent;
+  while (a >= 0) {
+    if (a == r) return true;
+    a = S.regions[a].parent;
+  }
+  return false;
+}
+
+/* Result of RegionBase::findEndOfStep. */
+struct StepEnd {
+  float t;        /* distance moved (<= s) */
+  int next;       /* next region; == current if no boundary; -1 left the chamber */
+  bool boundary;
+  float ownNx, ownNy, ownNz;   /* current shape's crossing normal (this step) */
+  int winner;                  /* child region whose shape won, -1 = own shape */
+  float winNx, winNy, winNz;
+};
+
+/* Displacement that carries a point across a boundary at coordinate scale |p|. */
+MD float nudge_nm(float px, float py, float pz) {
+  float m = fmaxf(fabsf(px), fmaxf(fabsf(py), fabsf(pz)));
+  return 1.0e-3f + 4.0e-6f * m;
+}
+
+/* RegionBase::findEndOfStep (RegionBase.cu:74-119) on the ray p + t d,
+ * 0 < t <= s.  A crossing of the chamber's own boundary leaves the chamber. */
+MD StepEnd find_end_of_step(const DevScene& S, int r, float px, float py, float pz, float dx, float dy, float dz,
+                            float s) {
+  StepEnd se;
+  const DRegion& rg = S.regions[r];
+  ShapeHit own = shape_first(S, rg.shape, px, py, pz, dx, dy, dz, s);
+  se.ownNx = own.nx;
+  se.ownNy = own.ny;
+  se.ownNz = own.nz;
+  float t = own.t;
+  int base = (t <= s && rg.parent >= 0) ? rg.parent : r;
+  se.winner = -1;
+  se.winNx = own.nx;
+  se.winNy = own.ny;
+  se.winNz = own.nz;
+  for (int c = 0; c < rg.n_children; ++c) {
+    int ch = S.children[rg.first_child + c];
+    ShapeHit h = shape_first(S, S.regions[ch].shape, px, py, pz, dx, dy, dz, fminf(s, t));
+    if (h.t <= s && h.t < t) {
+      t = h.t;
+      base = ch;
+      se.winner = ch;
+      se.winNx = h.nx;
+      se.winNy = h.ny;
+      se.winNz = h.nz;
+    }
+  }
+  if (t <= s) {
+    se.t = t;
+    se.boundary = true;
+    if (rg.parent < 0 && se.winner < 0) {
+      se.next = -1; /* through the chamber wall */
+      return se;
+    }
+    float qx = px + t * dx, qy = py + t * dy, qz = pz + t * dz;
+    float dd = nudge_nm(qx, qy, qz);
+    float ox = qx + dd * dx, oy = qy + dd * dy, oz = qz + dd * dz;
+    int res = -1;
+    for (int guard = 0; guard < 64 && base >= 0; ++guard) {
+      res = containing_sub_region(S, base, ox, oy, oz);
+      if (res >= 0) break;
+      base = S.regions[base].parent;
+    }
+    se.next = res;
+    return se;
+  }
+  se.t = s;
+  se.boundary = false;
+  se.next = r;
+  return se;
+}
+
+}  // namespace monsel_dev
+
+#endif
