@@ -1,0 +1,155 @@
+"""Benchmark: primaries/s of the electron-transport hot path on one MI355X.
+
+Workload (one "step"): the line scan of one trapezoidal PMMA line on a Si
+substrate -- height 50 nm, bottom width 20 nm, sidewall angle 3 deg, top
+corner radius 5 nm (NormalMultiPlaneShape n NormalCylindricalShape, built by
+NShapes::createLine) -- at 0.5 keV with a 0.5 nm Gaussian beam, 128 pixels at
+1 nm pitch across the line, 2000 primaries per pixel (256,000 primaries),
+FP32 transport, FP64 tallies.  The scene and tables are resident on the
+device before the timed region; each step submits the job through the C-ABI
+and waits for the tallies.
+
+Prints one JSON line.  `roofline` refers to the dominant kernel
+(k_transport): algorithmic bytes per electron step x steps / summed kernel
+time from HIP events on the engine's stream.  `cpu_baseline` times the CPU
+oracle (the FP64 restatement, one worker per host core up to 16) on a
+bounded sample of the same scan.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# algorithmic HBM bytes per electron step for this workload: electron state
+# read + written (2 x 48 B), elastic cross-section gathers (4 nodes x 4 B x
+# 3 elements), and the Mott angle gather (16 x 4 B) on ~70 % of steps, plus
+# escape tallies -- about 190 B/step for PMMA transport
+BYTES_PER_STEP = 190.0
+HBM_PEAK_GBS = 8000.0
+
+
+def build_scene():
+    from cudamonsel_amd import scenes
+    import numpy as np
+    sc = scenes.line_on_substrate(h_nm=50.0, w_nm=20.0, theta_deg=3.0, r_nm=5.0, length_nm=1000.0,
+                                  line="PMMA", substrate="Si", beam_eV=500.0, sigma_nm=0.5, compat=True)
+    xs = np.arange(-64.0, 64.0, 1.0)
+    centers = scenes.scan_centers(xs, [0.0], sc.beam_z)
+    return sc, centers
+
+
+def cpu_baseline(sc, centers, target_s=10.0):
+    """Time the oracle on a bounded sample of the scan (all 128 pixels, a few
+    primaries each), sized to about target_s seconds of wall time."""
+    import cudamonsel_amd as M
+    from oracle import oracle as O
+    cs = sc.monte.compileScene()
+    osc = O.OracleScene(cs)
+    threads = min(os.cpu_count() or 1, 16)
+    per = 4
+    t0 = time.time()
+    osc.run(centers, per, M.ToSI.eV(500.0), 0.5e-9, seed=5, nbins=50, threads=threads)
+    dt = max(time.time() - t0, 1e-3)
+    per = max(4, int(per * target_s / dt))
+    t0 = time.time()
+    o = osc.run(centers, per, M.ToSI.eV(500.0), 0.5e-9, seed=6, nbins=50, threads=threads)
+    dt = time.time() - t0
+    n = per * centers.shape[0]
+    return {"value": round(n / dt, 1), "unit": "primaries/s", "cores": threads, "kind": "port",
+            "sample": f"line scan, 128 pixels x {per} primaries ({n} primaries, {o['stats']['steps']} steps) "
+                      f"in {dt:.1f} s on the FP64 oracle"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--per-pixel", type=int, default=2000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="run one device copy of this size first (HBM-counter calibration under rocprof)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box here has one GPU and no multi-GPU path is built "
+              "(primaries shard by pixel or job across GPUs; see the design notes).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import cudamonsel_amd as M
+
+    if args.calib_copy_mb > 0:
+        import torch
+        a = torch.empty(args.calib_copy_mb << 20, dtype=torch.uint8, device="cuda:0")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+
+    sc, centers = build_scene()
+    monte = sc.monte
+    n_prim = centers.shape[0] * args.per_pixel
+    for _ in range(args.warmup):
+        monte.runPixels(centers, args.per_pixel, 50, 50.0)
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = monte.runPixels(centers, args.per_pixel, 50, 50.0)
+        stats.append(r.stats)
+    wall = time.perf_counter() - t0
+    steps_total = sum(s.steps for s in stats)
+    k_ms = sum(s.transport_ms for s in stats)
+    launches = sum(s.transport_launches for s in stats)
+    achieved = BYTES_PER_STEP * steps_total / (k_ms * 1e-3) / 1e9
+    se = r.se.sum() / n_prim
+    bse = (r.back.sum() - r.se.sum()) / n_prim
+    out = {
+        "metric": "primary electrons/s, 0.5 keV PMMA line on Si line scan",
+        "value": round(n_prim * args.steps / wall, 1),
+        "unit": "primaries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": "line scan: PMMA line h50 w20 nm, 3 deg, r5 nm on Si, 0.5 keV, sigma 0.5 nm, "
+                        "128 px x 2000 primaries",
+            "pixels": int(centers.shape[0]),
+            "primaries_per_pixel": args.per_pixel,
+            "primaries_per_step": n_prim,
+            "electron_steps_per_primary": round(steps_total / (n_prim * args.steps), 1),
+            "electron_steps_per_s": round(steps_total / wall, 1),
+            "images_per_hour": round(3600.0 * args.steps / wall, 1),
+            "transport_launches_per_step": launches / args.steps,
+            "mean_SE_yield": round(float(se), 4),
+            "mean_BSE_yield": round(float(bse), 4),
+            "ref_compat": "reference",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "k_transport",
+            "kernel_ms_per_launch": round(k_ms / max(launches, 1), 3),
+            "bytes_per_step": BYTES_PER_STEP,
+        },
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(sc, centers)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

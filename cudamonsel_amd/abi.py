@@ -35,8 +35,8 @@ class JobStats(ctypes.Structure):
     _fields_ = [
         ("primaries", ctypes.c_uint64), ("electrons", ctypes.c_uint64), ("steps", ctypes.c_uint64),
         ("escapes", ctypes.c_uint64), ("watchdog_drops", ctypes.c_uint64), ("queue_overflow", ctypes.c_uint64),
-        ("generations", ctypes.c_int32), ("pad0", ctypes.c_int32),
-        ("device_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+        ("generations", ctypes.c_int32), ("transport_launches", ctypes.c_int32),
+        ("device_ms", ctypes.c_double), ("total_ms", ctypes.c_double), ("transport_ms", ctypes.c_double),
     ]
 
 

@@ -1037,7 +1037,7 @@ struct monsel_ctx_s {
   /* job workspace (worker thread only) */
   DBuf q[2], qcount, prim, pix, hist, stats, centers;
   unsigned int* host_count = nullptr; /* pinned */
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, evk0 = nullptr, evk1 = nullptr;
   int blocks_per_cu = 0;
 };
 
@@ -1141,12 +1141,18 @@ void run_job(monsel_ctx_s* C, Job& job, SceneDev& sd) {
       G.out_u = (uint32_t*)(C->q[cur ^ 1].as<float>() + (size_t)kQF * cap);
       G.out_cap = cap;
       G.out_count = counts + 1;
+      HIP_TRY(hipEventRecord(C->evk0, C->stream));
       hipLaunchKernelGGL(k_transport, dim3(grid), dim3(block), 0, C->stream, sd.dev, J, G);
       HIP_TRY(hipGetLastError());
+      HIP_TRY(hipEventRecord(C->evk1, C->stream));
       HIP_TRY(hipMemcpyAsync(C->host_count, counts + 1, sizeof(unsigned int), hipMemcpyDeviceToHost, C->stream));
       HIP_TRY(hipStreamSynchronize(C->stream));
       uint32_t outc = *C->host_count;
       in_count = outc > cap ? cap : outc;
+      float kms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&kms, C->evk0, C->evk1));
+      job.stats.transport_ms += kms;
+      job.stats.transport_launches += 1;
       cur ^= 1;
       generations = std::max(generations, g + 1);
     }
@@ -1284,6 +1290,8 @@ int monsel_create(int device_ordinal, const monsel_tuning* tuning, monsel_ctx* o
     HIP_TRY(hipHostMalloc((void**)&C->host_count, 64, hipHostMallocDefault));
     HIP_TRY(hipEventCreate(&C->ev0));
     HIP_TRY(hipEventCreate(&C->ev1));
+    HIP_TRY(hipEventCreate(&C->evk0));
+    HIP_TRY(hipEventCreate(&C->evk1));
     int block = C->tuning.block_size > 0 ? C->tuning.block_size : 256;
     if (block % 64 != 0 || block > 1024) throw MonselError(MONSEL_E_INVALID, "block_size must be a multiple of 64, <= 1024");
     if (C->tuning.blocks_per_cu > 0) C->blocks_per_cu = C->tuning.blocks_per_cu;
@@ -1314,6 +1322,8 @@ int monsel_destroy(monsel_ctx ctx) {
   if (ctx->host_count) (void)hipHostFree(ctx->host_count);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->evk0) (void)hipEventDestroy(ctx->evk0);
+  if (ctx->evk1) (void)hipEventDestroy(ctx->evk1);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return MONSEL_OK;

@@ -222,6 +222,8 @@ PYBIND11_MODULE(_monsel, m) {
       .def_readonly("queue_overflow", &monsel_job_stats::queue_overflow)
       .def_readonly("generations", &monsel_job_stats::generations)
       .def_readonly("device_ms", &monsel_job_stats::device_ms)
+      .def_readonly("transport_ms", &monsel_job_stats::transport_ms)
+      .def_readonly("transport_launches", &monsel_job_stats::transport_launches)
       .def_readonly("total_ms", &monsel_job_stats::total_ms);
 
   py::class_<PixelResults>(m, "PixelResults")

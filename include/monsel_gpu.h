@@ -257,9 +257,10 @@ typedef struct monsel_job_stats {
   uint64_t watchdog_drops;
   uint64_t queue_overflow; /* secondaries lost to a full generation buffer */
   int32_t generations;
-  int32_t pad0;
+  int32_t transport_launches; /* launches of the transport kernel */
   double device_ms;       /* transport + reduction kernels */
   double total_ms;        /* submit .. results on device */
+  double transport_ms;    /* sum of transport-kernel durations (HIP events) */
 } monsel_job_stats;
 
 /* Launch shape; zero fields select defaults. */

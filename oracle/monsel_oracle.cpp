@@ -1769,8 +1769,13 @@ int oracle_run_job(void* scene, const monsel_job_desc* job, monsel_tallies* out,
   jp.seed = job->seed;
   int np = job->n_pixels;
   std::vector<PixelTally> px(np);
-  if (threads <= 0) threads = (int)std::thread::hardware_concurrency();
+  if (threads <= 0) {
+    /* default: the host's cores, at most 16 (one GPU's share of a shared box) */
+    threads = (int)std::thread::hardware_concurrency();
+    if (threads > 16) threads = 16;
+  }
   if (threads < 1) threads = 1;
+  if (threads > np) threads = np > 0 ? np : 1;
   std::vector<JobHist> hists(threads);
   std::vector<Counters> cnts(threads);
   for (auto& h : hists) h.init(jp.nbins);

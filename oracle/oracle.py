@@ -48,8 +48,8 @@ class JobStats(ctypes.Structure):
     _fields_ = [
         ("primaries", ctypes.c_uint64), ("electrons", ctypes.c_uint64), ("steps", ctypes.c_uint64),
         ("escapes", ctypes.c_uint64), ("watchdog_drops", ctypes.c_uint64), ("queue_overflow", ctypes.c_uint64),
-        ("generations", ctypes.c_int32), ("pad0", ctypes.c_int32),
-        ("device_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+        ("generations", ctypes.c_int32), ("transport_launches", ctypes.c_int32),
+        ("device_ms", ctypes.c_double), ("total_ms", ctypes.c_double), ("transport_ms", ctypes.c_double),
     ]
 
 
@@ -97,8 +97,13 @@ def uptr(a):
 
 
 class OracleScene:
-    def __init__(self, desc_address):
-        self.h = lib().oracle_scene_create(ctypes.c_void_p(desc_address))
+    """Oracle copy of a scene.  `compiled` is a cudamonsel_amd.CompiledScene
+    (kept referenced while the descriptor is read) or a descriptor address."""
+
+    def __init__(self, compiled):
+        self.compiled = compiled
+        addr = compiled if isinstance(compiled, int) else compiled.desc_address()
+        self.h = lib().oracle_scene_create(ctypes.c_void_p(addr))
         if not self.h:
             raise RuntimeError("oracle scene: " + lib().oracle_last_error().decode())
 
@@ -124,7 +129,7 @@ class OracleScene:
         rc = lib().oracle_run_job(self.h, ctypes.byref(job), ctypes.byref(t), ctypes.byref(st), threads)
         if rc != 0:
             raise RuntimeError("oracle_run_job failed")
-        out["stats"] = {k: getattr(st, k) for k, _ in JobStats._fields_ if k != "pad0"}
+        out["stats"] = {k: getattr(st, k) for k, _ in JobStats._fields_}
         out["perPixel"] = per_pixel
         return out
 

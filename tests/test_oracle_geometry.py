@@ -22,7 +22,7 @@ def line_scene(request, oracle):
     from cudamonsel_amd import scenes
     sc = scenes.line_on_substrate(h_nm=H, w_nm=W, theta_deg=TH, r_nm=R, compat=request.param)
     cs = sc.monte.compileScene()
-    return request.param, sc, cs, oracle.OracleScene(cs.desc_address())
+    return request.param, sc, cs, oracle.OracleScene(cs)
 
 
 def rounded_trapezoid(x, z, compat):
@@ -144,7 +144,7 @@ def test_chamber_sphere_reverse_sum(oracle):
     from cudamonsel_amd import scenes
     sc = scenes.flat("PMMA")
     cs = sc.monte.compileScene()
-    osc = oracle.OracleScene(cs.desc_address())
+    osc = oracle.OracleScene(cs)
     rng = np.random.default_rng(9)
     for _ in range(20):
         d = rng.normal(size=3)

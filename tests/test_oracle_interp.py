@@ -62,7 +62,7 @@ def poly_scene(tmp_path_factory, M):
 
 def _scene(oracle, sc):
     cs = sc.monte.compileScene()
-    return oracle.OracleScene(cs.desc_address()), cs
+    return oracle.OracleScene(cs), cs
 
 
 def test_nu_d1_d2_d3_exact_for_polynomials(oracle, poly_scene):

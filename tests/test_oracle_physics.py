@@ -46,7 +46,7 @@ def pmma(oracle):
     from cudamonsel_amd import scenes
     sc = scenes.flat("PMMA", beam_eV=500.0)
     cs = sc.monte.compileScene()
-    return sc, cs, oracle.OracleScene(cs.desc_address())
+    return sc, cs, oracle.OracleScene(cs)
 
 
 @pytest.fixture(scope="module")
@@ -54,7 +54,7 @@ def si(oracle):
     from cudamonsel_amd import scenes
     sc = scenes.flat("Si", beam_eV=1000.0)
     cs = sc.monte.compileScene()
-    return sc, cs, oracle.OracleScene(cs.desc_address())
+    return sc, cs, oracle.OracleScene(cs)
 
 
 def node_energies_eV():

@@ -1,0 +1,35 @@
+"""The oracle against the reference's own committed run log
+(CudaMONSEL/CudaMONSEL/outputs/output_release.txt, derived into
+tests/golden/reference_run_log.json by tests/golden/make_reference_run_log.py).
+
+The log is a 0.5 keV PMMA line scan with 100 primaries per pixel from an
+unrecorded revision of the reference; far from the line (y > 700 nm) the
+surface is flat PMMA.  Its BSE yield is compared within 4 standard errors,
+its SE yield -- counted as E <= 40 eV through the histogram defect -- within
+2 % (the oracle sits 1.3 % above the log, independent of the random-number
+generator model; see the design notes).  Smoke-level, as the log's revision
+is unknown; the 3-sigma gates are GPU against the oracle."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "reference_run_log.json")
+
+
+@pytest.mark.slow
+def test_flat_pmma_yields_match_reference_log(oracle, M):
+    g = json.load(open(GOLD))["flat_far"]
+    from cudamonsel_amd import scenes
+    sc = scenes.flat("PMMA", beam_eV=500.0)
+    osc = oracle.OracleScene(sc.monte.compileScene())
+    npx, per = 48, 500
+    centers = np.array([[0.0, 0.0, sc.beam_z]] * npx)
+    r = osc.run(centers, per, M.ToSI.eV(500.0), 0.5e-9, seed=99, nbins=50, threads=8)
+    n = npx * per
+    se = r["se"].sum() / n
+    bse = (r["back"].sum() - r["se"].sum()) / n
+    bse_err = np.sqrt((r["bseSq"].sum() / n - bse ** 2) / n)
+    assert abs(bse - g["bse_mean"]) < 4 * np.hypot(bse_err, g["bse_sem"])
+    assert abs(se / g["se_mean"] - 1.0) < 0.02
