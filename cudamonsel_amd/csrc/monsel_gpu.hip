@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -1089,7 +1090,10 @@ void run_job(monsel_ctx_s* C, Job& job, SceneDev& sd) {
   int grid = C->n_cu * (C->blocks_per_cu > 0 ? C->blocks_per_cu : 8);
   int maxGen = C->tuning.max_generations > 0 ? C->tuning.max_generations : 64;
   uint32_t k0 = (uint32_t)jd.seed, k1 = (uint32_t)(jd.seed >> 32);
-  HIP_TRY(hipEventRecord(C->ev0, C->stream));
+  /* Event markers time the kernels; MONSEL_NO_EVENT_TIMING=1 leaves them out
+   * (hardware-counter profiling sessions reject the marker packets). */
+  const bool timing = !getenv("MONSEL_NO_EVENT_TIMING");
+  if (timing) HIP_TRY(hipEventRecord(C->ev0, C->stream));
   int generations = 0;
   for (uint64_t start = 0; start < total; start += chunkMax) {
     uint32_t n = (uint32_t)std::min<uint64_t>(chunkMax, total - start);
@@ -1141,16 +1145,16 @@ void run_job(monsel_ctx_s* C, Job& job, SceneDev& sd) {
       G.out_u = (uint32_t*)(C->q[cur ^ 1].as<float>() + (size_t)kQF * cap);
       G.out_cap = cap;
       G.out_count = counts + 1;
-      HIP_TRY(hipEventRecord(C->evk0, C->stream));
+      if (timing) HIP_TRY(hipEventRecord(C->evk0, C->stream));
       hipLaunchKernelGGL(k_transport, dim3(grid), dim3(block), 0, C->stream, sd.dev, J, G);
       HIP_TRY(hipGetLastError());
-      HIP_TRY(hipEventRecord(C->evk1, C->stream));
+      if (timing) HIP_TRY(hipEventRecord(C->evk1, C->stream));
       HIP_TRY(hipMemcpyAsync(C->host_count, counts + 1, sizeof(unsigned int), hipMemcpyDeviceToHost, C->stream));
       HIP_TRY(hipStreamSynchronize(C->stream));
       uint32_t outc = *C->host_count;
       in_count = outc > cap ? cap : outc;
       float kms = 0.f;
-      HIP_TRY(hipEventElapsedTime(&kms, C->evk0, C->evk1));
+      if (timing) HIP_TRY(hipEventElapsedTime(&kms, C->evk0, C->evk1));
       job.stats.transport_ms += kms;
       job.stats.transport_launches += 1;
       cur ^= 1;
@@ -1163,7 +1167,7 @@ void run_job(monsel_ctx_s* C, Job& job, SceneDev& sd) {
                        bse_sq);
     HIP_TRY(hipGetLastError());
   }
-  HIP_TRY(hipEventRecord(C->ev1, C->stream));
+  if (timing) HIP_TRY(hipEventRecord(C->ev1, C->stream));
   /* results */
   job.back.resize(np);
   job.fwd.resize(np);
@@ -1189,7 +1193,7 @@ void run_job(monsel_ctx_s* C, Job& job, SceneDev& sd) {
   HIP_TRY(hipMemcpyAsync(st.data(), C->stats.p, 8 * 16, hipMemcpyDeviceToHost, C->stream));
   HIP_TRY(hipStreamSynchronize(C->stream));
   float ms = 0.f;
-  HIP_TRY(hipEventElapsedTime(&ms, C->ev0, C->ev1));
+  if (timing) HIP_TRY(hipEventElapsedTime(&ms, C->ev0, C->ev1));
   job.stats.primaries = total;
   job.stats.steps = st[kStatSteps];
   job.stats.electrons = st[kStatElectrons];
