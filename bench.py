@@ -75,8 +75,8 @@ def main():
                     help="run one device copy of this size first (HBM-counter calibration under rocprof)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box here has one GPU and no multi-GPU path is built "
-              "(primaries shard by pixel or job across GPUs; see the design notes).", file=sys.stderr)
+        print("bench.py measures one GPU: no multi-GPU path is built (a scan would shard by pixel "
+              "or by job across GPUs, one process per GPU).", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -104,7 +104,8 @@ def main():
     steps_total = sum(s.steps for s in stats)
     k_ms = sum(s.transport_ms for s in stats)
     launches = sum(s.transport_launches for s in stats)
-    achieved = BYTES_PER_STEP * steps_total / (k_ms * 1e-3) / 1e9
+    # kernel time is zero when event timing is switched off (counter sessions)
+    achieved = BYTES_PER_STEP * steps_total / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
     se = r.se.sum() / n_prim
     bse = (r.back.sum() - r.se.sum()) / n_prim
     out = {

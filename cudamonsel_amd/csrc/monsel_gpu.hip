@@ -43,6 +43,9 @@ using namespace monsel_dev;
 /* ===================================================================== */
 
 static thread_local std::string g_err;
+/* text of the HIP call in progress, named when a C++ exception escapes a
+ * runtime call (e.g. from an interposed profiling library) */
+static thread_local const char* g_hip_call = "";
 
 static int set_err(int st, const std::string& m) {
   g_err = m;
@@ -51,6 +54,7 @@ static int set_err(int st, const std::string& m) {
 
 #define HIP_TRY(expr)                                                                                        \
   do {                                                                                                       \
+    g_hip_call = #expr;                                                                                      \
     hipError_t _e = (expr);                                                                                  \
     if (_e != hipSuccess)                                                                                    \
       throw MonselError(MONSEL_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e));                \
@@ -68,6 +72,8 @@ struct MonselError {
 
 constexpr int kQF = 7; /* float fields per queued electron: p, d, E */
 constexpr int kQU = 4; /* u32 fields: region, primary, lineage lo/hi */
+constexpr int kEF = 7; /* float fields per escape record: tally origin, d, E */
+constexpr int kEU = 2; /* u32 fields: primary, through-the-chamber-wall flag */
 constexpr int kStatSteps = 0, kStatElectrons = 1, kStatEscapes = 2, kStatWatchdog = 3, kStatOverflow = 4;
 
 struct JobDev {
@@ -105,6 +111,11 @@ struct GenDev {
   uint32_t* out_u;
   uint32_t out_cap;
   unsigned int* out_count;
+  /* escape records, tallied by k_tally_escapes after the launch */
+  float* esc_f;
+  uint32_t* esc_u;
+  uint32_t esc_cap;
+  unsigned int* esc_count;
 };
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
@@ -113,7 +124,7 @@ __device__ __forceinline__ unsigned long long wave_ballot(bool p) { return __bal
 
 /* Histogram::bin with edges min + i*delta (Histogram.cu:73-79,
  * Algorithm.cu:5-14); compat = the reference's insertion-point semantics. */
-__device__ int hist_bin(double val, double min, double max, int nbins, bool compat) {
+__device__ __forceinline__ int hist_bin(double val, double min, double max, int nbins, bool compat) {
   const double delta = (max - min) / nbins;
   int l = 0, r = nbins;
   while (l <= r) {
@@ -129,7 +140,7 @@ __device__ int hist_bin(double val, double min, double max, int nbins, bool comp
 
 /* BackscatterEvent -> BackscatterStats::actionPerformed (BackscatterStats.cu:147-170)
  * at the chamber wall point reached from (px,py,pz) along (dx,dy,dz). */
-__device__ void tally_escape(const DevScene& S, const JobDev& J, const Elec& e, float px, float py, float pz,
+__device__ __forceinline__ void tally_escape(const DevScene& S, const JobDev& J, const Elec& e, float px, float py, float pz,
                              float dx, float dy, float dz, bool fromRoot) {
   double x = px, y = py, z = pz;
   if (fromRoot) {
@@ -173,7 +184,7 @@ __device__ void tally_escape(const DevScene& S, const JobDev& J, const Elec& e, 
 enum StepResult { STEP_CONTINUE = 0, STEP_DEAD = 1, STEP_ESCAPED = 2, STEP_WATCHDOG = 3 };
 
 /* One MonteCarloSS::takeStep (MonteCarloSS.cu:107-163). */
-__device__ int take_step(const DevScene& S, const JobDev& J, Elec& e, Spawn& sp) {
+__device__ __forceinline__ int take_step(const DevScene& S, const JobDev& J, Elec& e, Spawn& sp) {
   sp.has = false;
   int region = e.region;
   if (region < 0 || !shape_contains(S, S.regions[region].shape, e.px, e.py, e.pz)) {
@@ -182,20 +193,25 @@ __device__ int take_step(const DevScene& S, const JobDev& J, Elec& e, Spawn& sp)
     if (region < 0) return STEP_DEAD;
   }
   const DMSM& msm = S.msms[S.regions[region].msm];
-  /* MONSEL_MaterialScatterModel::randomMeanPathLength (:64-91) */
-  float cum[kMaxMech];
+  float u[kStepUniforms];
+  step_uniforms(e, J.k0, J.k1, u);
+  /* MONSEL_MaterialScatterModel::randomMeanPathLength (:64-91); each
+   * mechanism's rate is kept in a per-lane LDS slot for the scatter choice */
+  __shared__ float l_rate[kMaxMech][kLaneBlock];
+  const int tid = threadIdx.x;
   float total = 0.f;
   float freePath;
   const float maxFreePath = 2.f * S.chamber_r;
   if (msm.is_null) {
     freePath = 1.0e9f; /* NullMaterialScatterModel.cu:32-35: 1 m */
   } else {
-    for (int k = 0; k < kMaxMech; ++k) {
-      if (k < msm.n_mech) total += mech_rate(S, S.mechs[msm.mech0 + k], e.E);
-      cum[k] = total;
+    for (int k = 0; k < msm.n_mech; ++k) {
+      const float rk = mech_rate(S, S.mechs[msm.mech0 + k], e.E);
+      l_rate[k][tid] = rk;
+      total += rk;
     }
     if (total != 0.f) {
-      float fp = -__logf(rng_next(e, J.k0, J.k1)) / total;
+      float fp = -__logf(u[0]) / total;
       freePath = fp > maxFreePath ? maxFreePath : fp;
     } else {
       freePath = maxFreePath;
@@ -215,11 +231,19 @@ __device__ int take_step(const DevScene& S, const JobDev& J, Elec& e, Spawn& sp)
   if (se.next == region) {
     /* MONSEL_MaterialScatterModel::scatter (:93-134), rates at prevE */
     if (msm.is_null || total == 0.f) return STEP_CONTINUE;
-    float r = rng_next(e, J.k0, J.k1) * total;
+    const float r = u[1] * total;
+    /* first mechanism whose cumulative rate reaches r (MONSEL_MaterialScatterModel.cu:129-131);
+     * the running sum repeats the order of `total` */
     int idx = 0;
-    while (idx < msm.n_mech - 1 && cum[idx] < r) ++idx;
+    float rate = l_rate[0][tid];
+    float cum = rate;
+    while (idx < msm.n_mech - 1 && cum < r) {
+      ++idx;
+      rate = l_rate[idx][tid];
+      cum += rate;
+    }
     bool trapped = false;
-    sp = mech_scatter(S, S.mechs[msm.mech0 + idx], e, prevE, trapped, J.k0, J.k1);
+    sp = mech_scatter(S, S.mechs[msm.mech0 + idx], e, prevE, rate, trapped, u);
     if (sp.has) {
       /* trackSecondaryElectron (MonteCarloSS.cu:165-174) */
       if (!(sp.E > msm.minE)) sp.has = false;
@@ -290,7 +314,7 @@ __device__ int take_step(const DevScene& S, const JobDev& J, Elec& e, Spawn& sp)
                 tp = 1.f - ratio * ratio;
               }
             }
-            transmits = rng_next(e, J.k0, J.k1) < tp;
+            transmits = u[1] < tp;
           }
         }
         if (transmits) {
@@ -310,34 +334,40 @@ __device__ int take_step(const DevScene& S, const JobDev& J, Elec& e, Spawn& sp)
         }
       }
     }
-    /* MonteCarloSS.cu:146-148: nudge along the new direction, re-locate */
+    /* MonteCarloSS.cu:146-148: nudge along the new direction; the region is
+     * re-located by the check that opens the next step */
     float dd = nudge_nm(e.px, e.py, e.pz);
     e.px += dd * e.dx;
     e.py += dd * e.dy;
     e.pz += dd * e.dz;
-    if (e.region < 0 || !shape_contains(S, S.regions[e.region].shape, e.px, e.py, e.pz))
-      e.region = containing_sub_region(S, 0, e.px, e.py, e.pz);
     return STEP_CONTINUE;
   }
-  /* left the chamber: BackscatterEvent */
-  bool fromRoot = S.regions[region].parent < 0 && se.winner < 0;
-  if (fromRoot) tally_escape(S, J, e, px, py, pz, e.dx, e.dy, e.dz, true);
-  else tally_escape(S, J, e, e.px, e.py, e.pz, e.dx, e.dy, e.dz, false);
+  /* left the chamber: BackscatterEvent.  The escape is recorded for
+   * k_tally_escapes: through the chamber wall the tally point is recomputed
+   * in FP64 from the step's start, so that start replaces the position and
+   * e.region carries the flag. */
+  const bool fromRoot = S.regions[region].parent < 0 && se.winner < 0;
+  if (fromRoot) {
+    e.px = px;
+    e.py = py;
+    e.pz = pz;
+  }
+  e.region = fromRoot ? 1 : 0;
   return STEP_ESCAPED;
 }
 
-__device__ void make_primary(const DevScene& S, const JobDev& J, uint32_t idx, Elec& e) {
+__device__ __forceinline__ void make_primary(const DevScene& S, const JobDev& J, uint32_t idx, Elec& e) {
   uint32_t job_idx = J.chunk_start + idx;
   uint32_t pix = job_idx / (uint32_t)J.ppp;
   e.primary = J.prim_base + idx;
   e.lin_lo = 0;
   e.lin_hi = 0;
-  e.draws = 0;
   e.nchild = 0;
   e.steps = 0;
   /* GaussianBeam::createElectron (GaussianBeam.cu:52-65) */
-  float u1 = rng_next(e, J.k0, J.k1);
-  float u2 = rng_next(e, J.k0, J.k1);
+  uint32_t a0, a1, a2, a3;
+  philox4x32_10(kPrimaryBlock, e.lin_lo, e.lin_hi, e.primary, J.k0, J.k1, a0, a1, a2, a3);
+  const float u1 = u01(a0), u2 = u01(a1);
   float r = __builtin_sqrtf(-2.f * __logf(u1)) * J.width;
   float s, c;
   __sincosf(kTwoPI * u2, &s, &c);
@@ -352,10 +382,15 @@ __device__ void make_primary(const DevScene& S, const JobDev& J, uint32_t idx, E
   e.region = containing_sub_region(S, 0, e.px, e.py, e.pz);
 }
 
-__global__ void __launch_bounds__(256) k_transport(DevScene S, JobDev J, GenDev G) {
+/* waves per SIMD the register allocation of k_transport targets */
+#ifndef MONSEL_TRANSPORT_WAVES_PER_EU
+#define MONSEL_TRANSPORT_WAVES_PER_EU 4
+#endif
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MONSEL_TRANSPORT_WAVES_PER_EU)))
+k_transport(DevScene S, JobDev J, GenDev G) {
   Elec e;
   bool alive = false, done = false;
-  unsigned long long nsteps = 0, nelec = 0, nesc = 0, nwd = 0, nover = 0;
+  uint32_t nsteps = 0, nelec = 0, nesc = 0, nwd = 0, nover = 0;
   const uint32_t in_count = G.mode == 0 ? J.chunk_n : G.in_n;
   const unsigned long long lanemask_lt = (1ull << lane_id()) - 1ull;
   for (;;) {
@@ -385,7 +420,6 @@ __global__ void __launch_bounds__(256) k_transport(DevScene S, JobDev J, GenDev 
             e.primary = G.in_u[1 * cap + idx];
             e.lin_lo = G.in_u[2 * cap + idx];
             e.lin_hi = G.in_u[3 * cap + idx];
-            e.draws = 0;
             e.nchild = 0;
             e.steps = 0;
           }
@@ -432,6 +466,32 @@ __global__ void __launch_bounds__(256) k_transport(DevScene S, JobDev J, GenDev 
         }
       }
     }
+    /* append escape records: one atomic per wave */
+    const bool esc = alive && res == STEP_ESCAPED;
+    unsigned long long em = wave_ballot(esc);
+    if (em) {
+      int leader = __ffsll((long long)em) - 1;
+      uint32_t base = 0;
+      if (lane_id() == leader) base = atomicAdd(G.esc_count, (unsigned int)__popcll(em));
+      base = __shfl(base, leader);
+      if (esc) {
+        uint32_t slot = base + (uint32_t)__popcll(em & lanemask_lt);
+        if (slot < G.esc_cap) {
+          const uint32_t cap = G.esc_cap;
+          G.esc_f[0 * cap + slot] = e.px;
+          G.esc_f[1 * cap + slot] = e.py;
+          G.esc_f[2 * cap + slot] = e.pz;
+          G.esc_f[3 * cap + slot] = e.dx;
+          G.esc_f[4 * cap + slot] = e.dy;
+          G.esc_f[5 * cap + slot] = e.dz;
+          G.esc_f[6 * cap + slot] = e.E;
+          G.esc_u[0 * cap + slot] = e.primary;
+          G.esc_u[1 * cap + slot] = (uint32_t)e.region;
+        } else {
+          nover++;
+        }
+      }
+    }
     if (alive && res != STEP_CONTINUE) {
       alive = false;
       if (res == STEP_ESCAPED) nesc++;
@@ -439,37 +499,69 @@ __global__ void __launch_bounds__(256) k_transport(DevScene S, JobDev J, GenDev 
     }
   }
   /* flush counters: lane sums reduced across the wave, one atomic each */
+  unsigned long long wsteps = nsteps, welec = nelec, wesc = nesc, wwd = nwd, wover = nover;
   for (int off = 32; off > 0; off >>= 1) {
-    nsteps += __shfl_down(nsteps, off);
-    nelec += __shfl_down(nelec, off);
-    nesc += __shfl_down(nesc, off);
-    nwd += __shfl_down(nwd, off);
-    nover += __shfl_down(nover, off);
+    wsteps += __shfl_down(wsteps, off);
+    welec += __shfl_down(welec, off);
+    wesc += __shfl_down(wesc, off);
+    wwd += __shfl_down(wwd, off);
+    wover += __shfl_down(wover, off);
   }
   if (lane_id() == 0) {
-    atomicAdd(&J.stats[kStatSteps], nsteps);
-    atomicAdd(&J.stats[kStatElectrons], nelec);
-    if (nesc) atomicAdd(&J.stats[kStatEscapes], nesc);
-    if (nwd) atomicAdd(&J.stats[kStatWatchdog], nwd);
-    if (nover) atomicAdd(&J.stats[kStatOverflow], nover);
+    atomicAdd(&J.stats[kStatSteps], wsteps);
+    atomicAdd(&J.stats[kStatElectrons], welec);
+    if (wesc) atomicAdd(&J.stats[kStatEscapes], wesc);
+    if (wwd) atomicAdd(&J.stats[kStatWatchdog], wwd);
+    if (wover) atomicAdd(&J.stats[kStatOverflow], wover);
   }
 }
 
-/* Per-primary SE/BSE counts -> per-pixel yields and second moments. */
-__global__ void k_reduce_primaries(JobDev J, unsigned long long* back_px, unsigned long long* se_px, double* se_sq,
-                                   double* bse_sq) {
+/* Tallies of the escape records of one transport launch (grid-stride over
+ * the record count the launch left in *count). */
+__global__ void __launch_bounds__(256) k_tally_escapes(DevScene S, JobDev J, const float* ef, const uint32_t* eu,
+                                                       uint32_t cap, const unsigned int* count) {
+  const uint32_t n = min(*count, cap);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    Elec e;
+    e.E = ef[6 * cap + i];
+    e.primary = eu[i];
+    tally_escape(S, J, e, ef[i], ef[cap + i], ef[2 * cap + i], ef[3 * cap + i], ef[4 * cap + i], ef[5 * cap + i],
+                 eu[cap + i] != 0u);
+  }
+}
+
+/* Per-primary SE/BSE counts -> per-pixel sums of n and n^2 (exact u64).
+ * Primaries of one pixel are consecutive, so a wave whose 64 primaries share
+ * a pixel reduces in registers and issues one atomic per sum; a wave that
+ * straddles pixels falls back to per-lane atomics. */
+__global__ void k_reduce_primaries(JobDev J, unsigned long long* back_px, unsigned long long* se_px,
+                                   unsigned long long* se_sq, unsigned long long* bse_sq) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= J.chunk_n) return;
-  uint32_t c = J.prim_counts[i];
-  uint32_t nse = c & 0xFFFFu, nbse = c >> 16;
-  uint32_t pix = (J.chunk_start + i) / (uint32_t)J.ppp;
-  if (nse | nbse) {
-    atomicAdd(&back_px[pix], (unsigned long long)(nse + nbse));
-    if (nse) {
-      atomicAdd(&se_px[pix], (unsigned long long)nse);
-      atomicAdd(&se_sq[pix], (double)nse * (double)nse);
+  const bool valid = i < J.chunk_n;
+  uint32_t c = valid ? J.prim_counts[i] : 0u;
+  uint32_t iv = valid ? i : J.chunk_n - 1u;
+  unsigned long long nse = c & 0xFFFFu, nbse = c >> 16;
+  uint32_t pix = (J.chunk_start + iv) / (uint32_t)J.ppp;
+  unsigned long long vb = nse + nbse, vs = nse, vss = nse * nse, vbb = nbse * nbse;
+  const uint32_t pix0 = __shfl(pix, 0);
+  if (wave_ballot(pix != pix0) == 0ull) {
+    for (int off = 32; off > 0; off >>= 1) {
+      vb += __shfl_xor(vb, off);
+      vs += __shfl_xor(vs, off);
+      vss += __shfl_xor(vss, off);
+      vbb += __shfl_xor(vbb, off);
     }
-    if (nbse) atomicAdd(&bse_sq[pix], (double)nbse * (double)nbse);
+    if (lane_id() == 0) {
+      if (vb) atomicAdd(&back_px[pix0], vb);
+      if (vs) atomicAdd(&se_px[pix0], vs);
+      if (vss) atomicAdd(&se_sq[pix0], vss);
+      if (vbb) atomicAdd(&bse_sq[pix0], vbb);
+    }
+  } else if (vb) {
+    atomicAdd(&back_px[pix], vb);
+    if (vs) atomicAdd(&se_px[pix], vs);
+    if (vss) atomicAdd(&se_sq[pix], vss);
+    if (vbb) atomicAdd(&bse_sq[pix], vbb);
   }
 }
 
@@ -1036,7 +1128,7 @@ struct monsel_ctx_s {
   bool stop = false;
   std::thread worker;
   /* job workspace (worker thread only) */
-  DBuf q[2], qcount, prim, pix, hist, stats, centers;
+  DBuf q[2], esc, qcount, prim, pix, hist, stats, centers;
   unsigned int* host_count = nullptr; /* pinned */
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evk0 = nullptr, evk1 = nullptr;
   int blocks_per_cu = 0;
@@ -1062,9 +1154,9 @@ void run_job(monsel_ctx_s* C, Job& job, SceneDev& sd) {
   unsigned long long* back_px = C->pix.as<unsigned long long>();
   unsigned long long* fwd_px = back_px + np;
   unsigned long long* se_px = fwd_px + np;
-  double* se_sq = (double*)(se_px + np);
-  double* bse_sq = se_sq + np;
-  double* benergy = bse_sq + np;
+  unsigned long long* se_sq = se_px + np;
+  unsigned long long* bse_sq = se_sq + np;
+  double* benergy = (double*)(bse_sq + np);
   size_t hn = (size_t)2 * (nbins + 2) + 182 + 362;
   C->hist.ensure(hn * 8);
   unsigned long long* hback = C->hist.as<unsigned long long>();
@@ -1082,8 +1174,9 @@ void run_job(monsel_ctx_s* C, Job& job, SceneDev& sd) {
   /* chunking: queue memory per chunk bounded */
   const uint32_t K = secondaries_per_primary(E0eV);
   const size_t perEntry = (size_t)(kQF + kQU) * 4;
-  const size_t budget = (size_t)6 << 30; /* both generation buffers */
-  uint64_t chunkMax = budget / (2 * perEntry * K);
+  const size_t perEsc = (size_t)(kEF + kEU) * 4;
+  const size_t budget = (size_t)6 << 30; /* both generation buffers + escape records */
+  uint64_t chunkMax = budget / ((2 * perEntry + perEsc) * K);
   if (C->tuning.queue_capacity > 0) chunkMax = std::max<uint64_t>(1, (uint64_t)C->tuning.queue_capacity / K);
   if (chunkMax < 1024) chunkMax = 1024;
   int block = C->tuning.block_size > 0 ? C->tuning.block_size : 256;
@@ -1100,6 +1193,7 @@ void run_job(monsel_ctx_s* C, Job& job, SceneDev& sd) {
     uint32_t cap = (uint32_t)std::min<uint64_t>((uint64_t)n * K, 0xFFFFFFF0ull);
     C->q[0].ensure((size_t)cap * perEntry);
     C->q[1].ensure((size_t)cap * perEntry);
+    C->esc.ensure((size_t)cap * perEsc);
     C->prim.ensure((size_t)n * 4);
     HIP_TRY(hipMemsetAsync(C->prim.p, 0, (size_t)n * 4, C->stream));
     JobDev J;
@@ -1128,11 +1222,11 @@ void run_job(monsel_ctx_s* C, Job& job, SceneDev& sd) {
     J.stats = C->stats.as<unsigned long long>();
     J.centers = C->centers.as<double>();
     unsigned int* counts = C->qcount.as<unsigned int>();
-    /* counts[0]: claim counter, counts[1]: spawned secondaries */
+    /* counts[0]: claim counter, counts[1]: spawned secondaries, counts[2]: escapes */
     uint32_t in_count = n;
     int cur = 0;
     for (int g = 0; g < maxGen && in_count > 0; ++g) {
-      HIP_TRY(hipMemsetAsync(counts, 0, 2 * sizeof(unsigned int), C->stream));
+      HIP_TRY(hipMemsetAsync(counts, 0, 3 * sizeof(unsigned int), C->stream));
       GenDev G;
       memset(&G, 0, sizeof(G));
       G.mode = g == 0 ? 0 : 1;
@@ -1145,11 +1239,20 @@ void run_job(monsel_ctx_s* C, Job& job, SceneDev& sd) {
       G.out_u = (uint32_t*)(C->q[cur ^ 1].as<float>() + (size_t)kQF * cap);
       G.out_cap = cap;
       G.out_count = counts + 1;
+      G.esc_f = C->esc.as<float>();
+      G.esc_u = (uint32_t*)(C->esc.as<float>() + (size_t)kEF * cap);
+      G.esc_cap = cap;
+      G.esc_count = counts + 2;
       if (timing) HIP_TRY(hipEventRecord(C->evk0, C->stream));
+      g_hip_call = "launch k_transport";
       hipLaunchKernelGGL(k_transport, dim3(grid), dim3(block), 0, C->stream, sd.dev, J, G);
       HIP_TRY(hipGetLastError());
       if (timing) HIP_TRY(hipEventRecord(C->evk1, C->stream));
       HIP_TRY(hipMemcpyAsync(C->host_count, counts + 1, sizeof(unsigned int), hipMemcpyDeviceToHost, C->stream));
+      g_hip_call = "launch k_tally_escapes";
+      hipLaunchKernelGGL(k_tally_escapes, dim3(C->n_cu * 4), dim3(256), 0, C->stream, sd.dev, J, G.esc_f, G.esc_u,
+                         cap, (const unsigned int*)(counts + 2));
+      HIP_TRY(hipGetLastError());
       HIP_TRY(hipStreamSynchronize(C->stream));
       uint32_t outc = *C->host_count;
       in_count = outc > cap ? cap : outc;
@@ -1163,6 +1266,7 @@ void run_job(monsel_ctx_s* C, Job& job, SceneDev& sd) {
     /* generation limit: any electrons left are dropped and counted */
     if (in_count > 0) job.stats.queue_overflow += in_count;
     int rb = 256;
+    g_hip_call = "launch k_reduce_primaries";
     hipLaunchKernelGGL(k_reduce_primaries, dim3((n + rb - 1) / rb), dim3(rb), 0, C->stream, J, back_px, se_px, se_sq,
                        bse_sq);
     HIP_TRY(hipGetLastError());
@@ -1179,12 +1283,11 @@ void run_job(monsel_ctx_s* C, Job& job, SceneDev& sd) {
   job.hfwd.resize(nbins + 2);
   job.helev.resize(182);
   job.hazim.resize(362);
-  std::vector<unsigned long long> st(16);
+  std::vector<unsigned long long> st(16), sq(2 * (size_t)np);
   HIP_TRY(hipMemcpyAsync(job.back.data(), back_px, 8 * np, hipMemcpyDeviceToHost, C->stream));
   HIP_TRY(hipMemcpyAsync(job.fwd.data(), fwd_px, 8 * np, hipMemcpyDeviceToHost, C->stream));
   HIP_TRY(hipMemcpyAsync(job.se.data(), se_px, 8 * np, hipMemcpyDeviceToHost, C->stream));
-  HIP_TRY(hipMemcpyAsync(job.se_sq.data(), se_sq, 8 * np, hipMemcpyDeviceToHost, C->stream));
-  HIP_TRY(hipMemcpyAsync(job.bse_sq.data(), bse_sq, 8 * np, hipMemcpyDeviceToHost, C->stream));
+  HIP_TRY(hipMemcpyAsync(sq.data(), se_sq, 16 * (size_t)np, hipMemcpyDeviceToHost, C->stream));
   HIP_TRY(hipMemcpyAsync(job.back_energy.data(), benergy, 8 * np, hipMemcpyDeviceToHost, C->stream));
   HIP_TRY(hipMemcpyAsync(job.hback.data(), hback, 8 * (nbins + 2), hipMemcpyDeviceToHost, C->stream));
   HIP_TRY(hipMemcpyAsync(job.hfwd.data(), hfwd, 8 * (nbins + 2), hipMemcpyDeviceToHost, C->stream));
@@ -1192,6 +1295,10 @@ void run_job(monsel_ctx_s* C, Job& job, SceneDev& sd) {
   HIP_TRY(hipMemcpyAsync(job.hazim.data(), hazim, 8 * 362, hipMemcpyDeviceToHost, C->stream));
   HIP_TRY(hipMemcpyAsync(st.data(), C->stats.p, 8 * 16, hipMemcpyDeviceToHost, C->stream));
   HIP_TRY(hipStreamSynchronize(C->stream));
+  for (size_t p = 0; p < (size_t)np; ++p) {
+    job.se_sq[p] = (double)sq[p];
+    job.bse_sq[p] = (double)sq[np + p];
+  }
   float ms = 0.f;
   if (timing) HIP_TRY(hipEventElapsedTime(&ms, C->ev0, C->ev1));
   job.stats.primaries = total;
@@ -1231,7 +1338,7 @@ void worker_main(monsel_ctx_s* C) {
         msg = e.msg;
       } catch (const std::exception& e) {
         st = MONSEL_E_DEVICE;
-        msg = e.what();
+        msg = std::string(e.what()) + " (in " + g_hip_call + ")";
       }
     }
     {
@@ -1259,7 +1366,7 @@ int guarded(F&& f) {
   } catch (const MonselError& e) {
     return set_err(e.status, e.msg);
   } catch (const std::exception& e) {
-    return set_err(MONSEL_E_DEVICE, e.what());
+    return set_err(MONSEL_E_DEVICE, std::string(e.what()) + " (in " + g_hip_call + ")");
   }
 }
 
@@ -1297,7 +1404,8 @@ int monsel_create(int device_ordinal, const monsel_tuning* tuning, monsel_ctx* o
     HIP_TRY(hipEventCreate(&C->evk0));
     HIP_TRY(hipEventCreate(&C->evk1));
     int block = C->tuning.block_size > 0 ? C->tuning.block_size : 256;
-    if (block % 64 != 0 || block > 1024) throw MonselError(MONSEL_E_INVALID, "block_size must be a multiple of 64, <= 1024");
+    if (block % 64 != 0 || block > kLaneBlock)
+      throw MonselError(MONSEL_E_INVALID, "block_size must be a multiple of 64, <= 256");
     if (C->tuning.blocks_per_cu > 0) C->blocks_per_cu = C->tuning.blocks_per_cu;
     else {
       int occ = 0;

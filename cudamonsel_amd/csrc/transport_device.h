@@ -39,7 +39,8 @@ constexpr float kEcharge = 1.60217653e-19f;         /* J per eV */
 
 /* ----------------------------------------------------------- RNG ------- */
 
-/* Philox4x32-10 (Salmon et al., SC'11); stream layout as in the oracle. */
+/* Philox4x32-10 (Salmon et al., SC'11).  Counter words: (block, lineage lo,
+ * lineage hi, primary index); key: the job seed. */
 MD void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
                       uint32_t& o0, uint32_t& o1, uint32_t& o2, uint32_t& o3) {
 #pragma unroll
@@ -81,21 +82,37 @@ struct Elec {
   int region;
   uint32_t primary;
   uint32_t lin_lo, lin_hi;
-  uint32_t draws;
   uint32_t nchild;
   uint32_t steps;
-  uint32_t r0, r1, r2, r3;
 };
 
 MD float u01(uint32_t x) { return (float)(x >> 9) * 1.1920928955078125e-07f + 5.9604644775390625e-08f; }
 
-MD float rng_next(Elec& e, uint32_t k0, uint32_t k1) {
-  uint32_t slot = e.draws & 3u;
-  if (slot == 0) philox4x32_10(e.draws >> 2, e.lin_lo, e.lin_hi, e.primary, k0, k1, e.r0, e.r1, e.r2, e.r3);
-  uint32_t x = slot == 0 ? e.r0 : (slot == 1 ? e.r1 : (slot == 2 ? e.r2 : e.r3));
-  e.draws++;
-  return u01(x);
+/* Uniforms per electron step: Philox blocks 2s and 2s+1 of the electron's
+ * stream, s = its step count.  Every draw of a step has a fixed slot:
+ *   u[0] free path, u[1] mechanism choice or barrier transmission,
+ *   u[2..7] the scatter mechanism's draws (at most six). */
+constexpr int kStepUniforms = 8;
+
+MD void step_uniforms(const Elec& e, uint32_t k0, uint32_t k1, float* u) {
+  uint32_t a0, a1, a2, a3, b0, b1, b2, b3;
+  philox4x32_10(2u * e.steps, e.lin_lo, e.lin_hi, e.primary, k0, k1, a0, a1, a2, a3);
+  philox4x32_10(2u * e.steps + 1u, e.lin_lo, e.lin_hi, e.primary, k0, k1, b0, b1, b2, b3);
+  u[0] = u01(a0);
+  u[1] = u01(a1);
+  u[2] = u01(a2);
+  u[3] = u01(a3);
+  u[4] = u01(b0);
+  u[5] = u01(b1);
+  u[6] = u01(b2);
+  u[7] = u01(b3);
 }
+
+/* block reserved for the primary's initial position (never a step block) */
+constexpr uint32_t kPrimaryBlock = 0xFFFFFFFFu;
+
+/* Lanes per block of every kernel that uses the per-lane LDS arrays below. */
+constexpr int kLaneBlock = 256;
 
 /* ----------------------------------------------------------- math ------ */
 
@@ -317,60 +334,104 @@ MD int nu_locate(const float* xs, int n, int order, float x, int& nearest) {
   return first;
 }
 
-/* Lagrange polynomial through (xs[k], f[k]), k = 0..order (order <= 3). */
-MD float nu_poly(const float* xs, const float* f, int order, float x) {
-  float y = 0.f;
-  for (int i = 0; i <= order; ++i) {
-    float w = 1.f;
-    for (int j = 0; j <= order; ++j)
-      if (j != i) w *= (x - xs[j]) / (xs[i] - xs[j]);
-    y += w * f[i];
+/* Lagrange basis weights of the ORDER+1 abscissae xs[0..ORDER] at x: the
+ * interpolant sum_i w_i f_i is the polynomial Neville's scheme
+ * (NULagrangeInterpolation.cu:85-112) evaluates.  Weights are formed once
+ * per axis and shared by every row of a multi-dimensional window. */
+template <int ORDER>
+MD void nu_weights(const float* xs, float x, float* w) {
+  /* differences are taken in units of the window's span, so that products
+   * of ORDER of them stay far from FP32 underflow whatever the grid's units
+   * (energies in J are ~1e-17) */
+  float xv[ORDER + 1];
+#pragma unroll
+  for (int i = 0; i <= ORDER; ++i) xv[i] = xs[i];
+  const float inv = 1.f / (xv[ORDER] - xv[0]);
+#pragma unroll
+  for (int i = 0; i <= ORDER; ++i) {
+    float num = 1.f, den = 1.f;
+#pragma unroll
+    for (int j = 0; j <= ORDER; ++j)
+      if (j != i) {
+        num *= (x - xv[j]) * inv;
+        den *= (xv[i] - xv[j]) * inv;
+      }
+    w[i] = num / den;
   }
+}
+
+template <int ORDER>
+MD float nu_dot(const float* w, const float* f) {
+  float y = 0.f;
+#pragma unroll
+  for (int i = 0; i <= ORDER; ++i) y += w[i] * f[i];
   return y;
 }
 
-MD float nu_d1(const float* f, const float* xs, int n, int order, float x) {
+template <int ORDER>
+MD float nu_poly(const float* xs, const float* f, float x) {
+  float w[ORDER + 1];
+  nu_weights<ORDER>(xs, x, w);
+  return nu_dot<ORDER>(w, f);
+}
+
+template <int ORDER>
+MD float nu_d1(const float* f, const float* xs, int n, float x) {
   int nearest;
-  int i0 = nu_locate(xs, n, order, x, nearest);
-  return nu_poly(xs + i0, f + i0, order, x);
+  int i0 = nu_locate(xs, n, ORDER, x, nearest);
+  return nu_poly<ORDER>(xs + i0, f + i0, x);
 }
 
 /* NULagrangeInterpolation::d2 (:169-192); f is an [na][nb] block. */
-MD float nu_d2(const float* f, const float* xa, int na, const float* xb, int nb, int order, float x0, float x1) {
+template <int ORDER>
+MD float nu_d2(const float* f, const float* xa, int na, const float* xb, int nb, float x0, float x1) {
   int nearest;
-  int i0 = nu_locate(xa, na, order, x0, nearest);
-  int j0 = nu_locate(xb, nb, order, x1, nearest);
-  float y[4];
-  for (int i = 0; i <= order; ++i) y[i] = nu_poly(xb + j0, f + (size_t)(i0 + i) * nb + j0, order, x1);
-  return nu_poly(xa + i0, y, order, x0);
+  int i0 = nu_locate(xa, na, ORDER, x0, nearest);
+  int j0 = nu_locate(xb, nb, ORDER, x1, nearest);
+  float wb[ORDER + 1], y[ORDER + 1];
+  nu_weights<ORDER>(xb + j0, x1, wb);
+#pragma unroll
+  for (int i = 0; i <= ORDER; ++i) y[i] = nu_dot<ORDER>(wb, f + (size_t)(i0 + i) * nb + j0);
+  return nu_poly<ORDER>(xa + i0, y, x0);
 }
 
-/* NULagrangeInterpolation::d3 (:215-240); compat_d3 reproduces line 236. */
-MD float nu_d3(const DevScene& S, const DNUTable& t, int order, float x0, float x1, float x2, bool compat_d3) {
+/* NULagrangeInterpolation::d3 (:215-240); compat_d3 reproduces line 236
+ * (outer abscissae taken from the start of the grid). */
+template <int ORDER>
+MD float nu_d3(const DevScene& S, const DNUTable& t, float x0, float x1, float x2, bool compat_d3) {
   const float* xa = S.nu_data + t.xoff[0];
   const float* xb = S.nu_data + t.xoff[1];
   const float* xc = S.nu_data + t.xoff[2];
   const float* v = S.nu_data + t.voff;
   int nearest;
-  int i0 = nu_locate(xa, t.n[0], order, x0, nearest);
-  int j0 = nu_locate(xb, t.n[1], order, x1, nearest);
-  int k0 = nu_locate(xc, t.n[2], order, x2, nearest);
-  float y[4];
-  for (int i = 0; i <= order; ++i) {
-    float z[4];
+  int i0 = nu_locate(xa, t.n[0], ORDER, x0, nearest);
+  int j0 = nu_locate(xb, t.n[1], ORDER, x1, nearest);
+  int k0 = nu_locate(xc, t.n[2], ORDER, x2, nearest);
+  float wb[ORDER + 1], wc[ORDER + 1], y[ORDER + 1];
+  nu_weights<ORDER>(xb + j0, x1, wb);
+  nu_weights<ORDER>(xc + k0, x2, wc);
+#pragma unroll 1
+  for (int i = 0; i <= ORDER; ++i) {
+    float z[ORDER + 1];
     const float* blk = v + (size_t)(i0 + i) * t.n[1] * t.n[2];
-    for (int j = 0; j <= order; ++j) z[j] = nu_poly(xc + k0, blk + (size_t)(j0 + j) * t.n[2] + k0, order, x2);
-    y[i] = nu_poly(xb + j0, z, order, x1);
+#pragma unroll
+    for (int j = 0; j <= ORDER; ++j) z[j] = nu_dot<ORDER>(wc, blk + (size_t)(j0 + j) * t.n[2] + k0);
+    const float yi = nu_dot<ORDER>(wb, z);
+#pragma unroll
+    for (int q = 0; q <= ORDER; ++q)
+      if (q == i) y[q] = yi;
   }
-  return nu_poly(compat_d3 ? xa : xa + i0, y, order, x0);
+  return nu_poly<ORDER>(compat_d3 ? xa : xa + i0, y, x0);
 }
 
-MD float nu_d4(const DevScene& S, const DNUTable& t, int order, const float* x, bool compat_d3) {
+template <int ORDER>
+MD float nu_d4(const DevScene& S, const DNUTable& t, const float* x, bool compat_d3) {
   const float* xa = S.nu_data + t.xoff[0];
   int nearest;
-  int i0 = nu_locate(xa, t.n[0], order, x[0], nearest);
-  float y[4];
-  for (int i = 0; i <= order; ++i) {
+  int i0 = nu_locate(xa, t.n[0], ORDER, x[0], nearest);
+  float y[ORDER + 1];
+#pragma unroll
+  for (int i = 0; i <= ORDER; ++i) {
     DNUTable sub;
     sub.dim = 3;
     for (int k = 0; k < 3; ++k) {
@@ -378,21 +439,46 @@ MD float nu_d4(const DevScene& S, const DNUTable& t, int order, const float* x, 
       sub.xoff[k] = t.xoff[k + 1];
     }
     sub.voff = t.voff + (i0 + i) * t.n[1] * t.n[2] * t.n[3];
-    y[i] = nu_d3(S, sub, order, x[1], x[2], x[3], compat_d3);
+    y[i] = nu_d3<ORDER>(S, sub, x[1], x[2], x[3], compat_d3);
   }
-  return nu_poly(xa + i0, y, order, x[0]);
+  return nu_poly<ORDER>(xa + i0, y, x[0]);
 }
 
-MD float nu_interp(const DevScene& S, int table, int order, const float* x, bool compat_d3) {
+template <int ORDER>
+MD float nu_interp_o(const DevScene& S, int table, const float* x, bool compat_d3) {
   const DNUTable& t = S.nu[table];
   const float* v = S.nu_data + t.voff;
   switch (t.dim) {
-    case 1: return nu_d1(v, S.nu_data + t.xoff[0], t.n[0], order, x[0]);
-    case 2: return nu_d2(v, S.nu_data + t.xoff[0], t.n[0], S.nu_data + t.xoff[1], t.n[1], order, x[0], x[1]);
-    case 3: return nu_d3(S, t, order, x[0], x[1], x[2], compat_d3);
-    case 4: return nu_d4(S, t, order, x, compat_d3);
+    case 1: return nu_d1<ORDER>(v, S.nu_data + t.xoff[0], t.n[0], x[0]);
+    case 2: return nu_d2<ORDER>(v, S.nu_data + t.xoff[0], t.n[0], S.nu_data + t.xoff[1], t.n[1], x[0], x[1]);
+    case 3: return nu_d3<ORDER>(S, t, x[0], x[1], x[2], compat_d3);
+    case 4: return nu_d4<ORDER>(S, t, x, compat_d3);
   }
   return 0.f;
+}
+
+MD float nu_interp(const DevScene& S, int table, int order, const float* x, bool compat_d3) {
+  if (order == 1) return nu_interp_o<1>(S, table, x, compat_d3);
+  if (order == 2) return nu_interp_o<2>(S, table, x, compat_d3);
+  return nu_interp_o<3>(S, table, x, compat_d3);
+}
+
+/* Lookups in tables whose dimension the scene compiler has checked (the
+ * inelastic mean free path is 1-D, the energy-loss and SE-energy tables
+ * 2-D, the angle table 3-D). */
+template <int ORDER>
+MD float nu_table1(const DevScene& S, int table, float x0) {
+  const DNUTable& t = S.nu[table];
+  return nu_d1<ORDER>(S.nu_data + t.voff, S.nu_data + t.xoff[0], t.n[0], x0);
+}
+template <int ORDER>
+MD float nu_table2(const DevScene& S, int table, float x0, float x1) {
+  const DNUTable& t = S.nu[table];
+  return nu_d2<ORDER>(S.nu_data + t.voff, S.nu_data + t.xoff[0], t.n[0], S.nu_data + t.xoff[1], t.n[1], x0, x1);
+}
+template <int ORDER>
+MD float nu_table3(const DevScene& S, int table, float x0, float x1, float x2, bool compat_d3) {
+  return nu_d3<ORDER>(S, S.nu[table], x0, x1, x2, compat_d3);
 }
 
 /* ------------------------------------------------------ mechanisms ----- */
@@ -430,7 +516,7 @@ MD float mech_rate(const DevScene& S, const DMech& m, float EeV) {
     case MONSEL_MECH_TABULATED_INELASTIC: { /* TabulatedInelasticSM.cu:436-459 */
       float kE = (EeV + m.p[TP_OFFSET]) * kEcharge;
       if (kE < m.p[TP_IIMFPDOM0]) return 0.f;
-      return m.p[TP_RATEMULT] * nu_interp(S, m.table[0], 1, &kE, false) * 1.0e-9f;
+      return m.p[TP_RATEMULT] * nu_table1<1>(S, m.table[0], kE) * 1.0e-9f;
     }
   }
   return 0.f;
@@ -501,7 +587,7 @@ MD void isotropic_dir(float u1, float u2, float& dx, float& dy, float& dz) {
 
 /* TabulatedInelasticSM::scatter (TabulatedInelasticSM.cu:209-434).  Table
  * lookups in SI units (J), results converted back to eV. */
-MD Spawn tabulated_scatter(const DevScene& S, const DMech& m, Elec& e, uint32_t k0, uint32_t k1) {
+MD Spawn tabulated_scatter(const DevScene& S, const DMech& m, Elec& e, const float* u) {
   Spawn se;
   se.has = false;
   const bool cd3 = (S.compat & MONSEL_COMPAT_D3_ABSCISSAE) != 0;
@@ -509,19 +595,17 @@ MD Spawn tabulated_scatter(const DevScene& S, const DMech& m, Elec& e, uint32_t 
   float kE0 = e.E * J;
   float kE = kE0 + m.p[TP_OFFSET] * J;
   if (kE < m.p[TP_EIDOM0]) return se;
-  float rnd0 = rng_next(e, k0, k1), rnd1 = rng_next(e, k0, k1), rnd2 = rng_next(e, k0, k1), rnd3 = rng_next(e, k0, k1);
-  float in[3] = {kE, rnd0, 0.f};
-  float deltaE = kE * nu_interp(S, m.table[1], 3, in, cd3);
+  const float rnd0 = u[2], rnd1 = u[3], rnd2 = u[4], rnd3 = u[5];
+  float deltaE = kE * nu_table2<3>(S, m.table[1], kE, rnd0);
   float gap = m.p[TP_GAP] * J, bandgap = m.p[TP_BANDGAP] * J;
   if (deltaE < gap && deltaE > 0.95f * gap) deltaE = gap;
   float d0x = e.dx, d0y = e.dy, d0z = e.dz;
   float theta = 0.f, sphi = 0.f, cphi = 1.f, phi = 0.f;
   float offsetFermi = m.p[TP_OFFSETFERMI] * J;
   if (deltaE >= bandgap) {
-    in[1] = (deltaE - gap) / (kE - offsetFermi - 2.f * gap);
-    in[1] = in[1] > 1.f ? 1.f : (in[1] < 0.f ? 0.f : in[1]);
-    in[2] = rnd1;
-    theta = nu_interp(S, m.table[2], 3, in, cd3);
+    float red = (deltaE - gap) / (kE - offsetFermi - 2.f * gap);
+    red = red > 1.f ? 1.f : (red < 0.f ? 0.f : red);
+    theta = nu_table3<3>(S, m.table[2], kE, red, rnd1, cd3);
     phi = kTwoPI * rnd2;
     __sincosf(phi, &sphi, &cphi);
     float st, ct;
@@ -546,8 +630,7 @@ MD Spawn tabulated_scatter(const DevScene& S, const DMech& m, Elec& e, uint32_t 
     rotate_dir(sx, sy, sz, ct, st, -cphi, -sphi);
   };
   auto se0 = [&](float dE) {
-    float in2[2] = {dE, rnd3};
-    float e0 = nu_interp(S, m.table[3], 3, in2, cd3);
+    float e0 = nu_table2<3>(S, m.table[3], dE, rnd3);
     if (e0 < m.p[TP_SE0RANGE0] * J) e0 = m.p[TP_SE0RANGE0] * J;
     else if (e0 > m.p[TP_SE0RANGE1] * J) e0 = m.p[TP_SE0RANGE1] * J;
     return e0;
@@ -567,8 +650,7 @@ MD Spawn tabulated_scatter(const DevScene& S, const DMech& m, Elec& e, uint32_t 
     if (be > 0.f) {
       energySE = (deltaE + bEref) - be;
       if (energySE + cb < minEgen) return se;
-      float u1 = rng_next(e, k0, k1), u2 = rng_next(e, k0, k1);
-      isotropic_dir(u1, u2, se.dx, se.dy, se.dz);
+      isotropic_dir(u[6], u[7], se.dx, se.dy, se.dz);
     } else {
       float root = 2.f * sqrtf(offsetFermi * (offsetFermi + deltaE));
       float sum = (2.f * offsetFermi) + deltaE;
@@ -594,7 +676,7 @@ MD Spawn tabulated_scatter(const DevScene& S, const DMech& m, Elec& e, uint32_t 
         float wx, wy, wz;
         dir_from_angles(ct, st, -cphi, -sphi, wx, wy, wz);
         float sp2, cp2, st2, ct2;
-        __sincosf(kTwoPI * rng_next(e, k0, k1), &sp2, &cp2);
+        __sincosf(kTwoPI * u[6], &sp2, &cp2);
         __sincosf(thSE, &st2, &ct2);
         rotate_dir(wx, wy, wz, ct2, st2, cp2, sp2);
         float wst = __builtin_sqrtf(fmaxf(0.f, wx * wx + wy * wy));
@@ -610,8 +692,7 @@ MD Spawn tabulated_scatter(const DevScene& S, const DMech& m, Elec& e, uint32_t 
       } else {
         energySE = (deltaE + se0(deltaE)) - offset;
         if (energySE + cb < minEgen) return se;
-        float u1 = rng_next(e, k0, k1), u2 = rng_next(e, k0, k1);
-        isotropic_dir(u1, u2, se.dx, se.dy, se.dz);
+        isotropic_dir(u[6], u[7], se.dx, se.dy, se.dz);
       }
     }
   }
@@ -620,46 +701,46 @@ MD Spawn tabulated_scatter(const DevScene& S, const DMech& m, Elec& e, uint32_t 
   return se;
 }
 
-/* ScatterMechanism::scatter */
-MD Spawn mech_scatter(const DevScene& S, const DMech& m, Elec& e, float prevE, bool& trapped, uint32_t k0,
-                      uint32_t k1) {
+/* ScatterMechanism::scatter.  `rate` is this mechanism's scatterRate at
+ * prevE, as summed when the step's free path was drawn. */
+MD Spawn mech_scatter(const DevScene& S, const DMech& m, Elec& e, float prevE, float rate, bool& trapped,
+                      const float* u) {
   Spawn se;
   se.has = false;
   switch (m.kind) {
     case MONSEL_MECH_NISTMOTT_ELASTIC:
     case MONSEL_MECH_BROWNING_ELASTIC:
     case MONSEL_MECH_RUTHERFORD_ELASTIC: { /* SelectableElasticSM.cu:49-67 */
-      float cum[kMaxElem];
-      float tot = 0.f;
-      for (int i = 0; i < m.nelem; ++i) {
-        const DElem& el = S.elems[m.elem0 + i];
-        tot += elem_sigma(S, el, prevE) * el.natoms;
-        cum[i] = tot;
-      }
-      float r = rng_next(e, k0, k1) * tot;
+      /* first element whose running sum of sigma * atoms reaches u * rate;
+       * the sum is re-formed in the order elastic_rate formed it */
+      const float r = u[2] * rate;
       int idx = 0;
-      while (idx < m.nelem - 1 && cum[idx] < r) ++idx;
+      float cum = elem_sigma(S, S.elems[m.elem0], prevE) * S.elems[m.elem0].natoms;
+      while (idx < m.nelem - 1 && cum < r) {
+        ++idx;
+        const DElem& el = S.elems[m.elem0 + idx];
+        cum += elem_sigma(S, el, prevE) * el.natoms;
+      }
       const DElem& el = S.elems[m.elem0 + idx];
       float ca;
       if (el.kind == MONSEL_MECH_BROWNING_ELASTIC ||
           (el.kind == MONSEL_MECH_NISTMOTT_ELASTIC && prevE < el.extrap)) {
-        float r1 = rng_next(e, k0, k1), r2 = rng_next(e, k0, k1);
-        ca = browning_cos(el, prevE, r1, r2);
+        ca = browning_cos(el, prevE, u[3], u[4]);
       } else if (el.kind == MONSEL_MECH_RUTHERFORD_ELASTIC || prevE >= kMaxNISTMott) {
-        ca = rutherford_cos(el, prevE, rng_next(e, k0, k1));
+        ca = rutherford_cos(el, prevE, u[3]);
       } else {
-        ca = mott_cos(S, el.mott, prevE, rng_next(e, k0, k1));
+        ca = mott_cos(S, el.mott, prevE, u[3]);
       }
       ca = fminf(1.f, fmaxf(-1.f, ca));
       float sa = __builtin_sqrtf(fmaxf(0.f, 1.f - ca * ca));
       float sb, cb;
-      __sincosf(kTwoPI * rng_next(e, k0, k1), &sb, &cb);
+      __sincosf(kTwoPI * u[5], &sb, &cb);
       rotate_dir(e.dx, e.dy, e.dz, ca, sa, cb, sb);
       return se;
     }
     case MONSEL_MECH_FITTED_INELASTIC: { /* FittedInelSM.cu:13-18 */
-      float u_phi = rng_next(e, k0, k1);
-      float u_th = rng_next(e, k0, k1);
+      float u_phi = u[2];
+      float u_th = u[3];
       isotropic_dir(u_th, u_phi, se.dx, se.dy, se.dz);
       se.E = m.p[1];
       se.has = true;
@@ -673,7 +754,7 @@ MD Spawn mech_scatter(const DevScene& S, const DMech& m, Elec& e, float prevE, b
       float kE0 = e.E;
       if (kE0 < m.p[0]) return se;
       float x = m.p[0] / kE0;
-      float r0 = rng_next(e, k0, k1), r1 = rng_next(e, k0, k1);
+      const float r0 = u[2], r1 = u[3];
       float ct;
       if (x < 0.1f) ct = 1.f + ((x * x) - (exp2f(4.f * r0) * powf(x, 2.f - 2.f * r0))) / 8.f;
       else {
@@ -690,7 +771,7 @@ MD Spawn mech_scatter(const DevScene& S, const DMech& m, Elec& e, float prevE, b
       return se;
     }
     case MONSEL_MECH_TABULATED_INELASTIC:
-      return tabulated_scatter(S, m, e, k0, k1);
+      return tabulated_scatter(S, m, e, u);
   }
   return se;
 }
@@ -874,67 +955,69 @@ struct ShapeHit {
   float nx, ny, nz;
 };
 
-/* First boundary of the composite shape along p + t d, 0 < t <= tmax. */
+/* First boundary of the composite shape along p + t d, 0 < t <= tmax.
+ * Each primitive's entry/exit parameters go to per-lane LDS slots; the walk
+ * then visits the crossings in increasing t, toggling the primitives'
+ * inside bits, until the truth table changes value.  Requires
+ * blockDim.x <= kLaneBlock. */
 MD ShapeHit shape_first(const DevScene& S, int s, float px, float py, float pz, float dx, float dy, float dz,
                         float tmax) {
+  __shared__ float l_tin[kMaxPrims][kLaneBlock];
+  __shared__ float l_tout[kMaxPrims][kLaneBlock];
+  __shared__ int8_t l_fin[kMaxPrims][kLaneBlock];
+  __shared__ int8_t l_fout[kMaxPrims][kLaneBlock];
+  const int tid = threadIdx.x;
   const DShape& sh = S.shapes[s];
   ShapeHit h;
   h.t = kInf;
   h.nx = h.ny = h.nz = 0.f;
-  float tin[kMaxPrims], tout[kMaxPrims];
-  int fin[kMaxPrims], fout[kMaxPrims];
   uint32_t bits = 0;
   const int n = sh.n_prims;
-#pragma unroll
-  for (int k = 0; k < kMaxPrims; ++k) {
-    tin[k] = -kInf;
-    tout[k] = -kInf;
-    fin[k] = fout[k] = -4;
-    if (k < n) {
-      Interval iv = prim_interval(S, S.prims[sh.first_prim + k], px, py, pz, dx, dy, dz);
-      if (!(iv.tin < iv.tout)) {
-        iv.tin = -kInf;
-        iv.tout = -kInf;
-      }
-      tin[k] = iv.tin;
-      tout[k] = iv.tout;
-      fin[k] = iv.fin;
-      fout[k] = iv.fout;
-      if (iv.tin <= 0.f && iv.tout > 0.f) bits |= 1u << k;
+  for (int k = 0; k < n; ++k) {
+    Interval iv = prim_interval(S, S.prims[sh.first_prim + k], px, py, pz, dx, dy, dz);
+    if (!(iv.tin < iv.tout)) {
+      iv.tin = -kInf;
+      iv.tout = -kInf;
     }
+    l_tin[k][tid] = iv.tin;
+    l_tout[k][tid] = iv.tout;
+    l_fin[k][tid] = (int8_t)iv.fin;
+    l_fout[k][tid] = (int8_t)iv.fout;
+    if (iv.tin <= 0.f && iv.tout > 0.f) bits |= 1u << k;
   }
   bool state = truth(sh, bits);
   float tcur = 0.f;
-  for (int it = 0; it < 2 * kMaxPrims; ++it) {
+  for (int it = 0; it < 2 * n; ++it) {
+    /* next crossing after tcur and the primitives crossing there */
     float tnext = kInf;
-#pragma unroll
-    for (int k = 0; k < kMaxPrims; ++k) {
-      float nk = tin[k] > tcur ? tin[k] : (tout[k] > tcur ? tout[k] : kInf);
-      tnext = fminf(tnext, nk);
-    }
-    if (!(tnext <= tmax)) return h;
-    int first = -1, firstFace = -4, second = -1, secondFace = -4;
-#pragma unroll
-    for (int k = 0; k < kMaxPrims; ++k) {
-      bool isIn = tin[k] == tnext && tin[k] > tcur;
-      bool isOut = !isIn && tout[k] == tnext && tout[k] > tcur;
-      if (isIn || isOut) {
-        bits ^= 1u << k;
-        int f = isIn ? fin[k] : fout[k];
-        if (first < 0) { first = k; firstFace = f; }
-        else if (second < 0) { second = k; secondFace = f; }
+    uint32_t mask = 0;
+    for (int k = 0; k < n; ++k) {
+      const float a = l_tin[k][tid], b = l_tout[k][tid];
+      const float nk = a > tcur ? a : (b > tcur ? b : kInf);
+      if (nk < tnext) {
+        tnext = nk;
+        mask = 1u << k;
+      } else if (nk == tnext) {
+        mask |= 1u << k;
       }
     }
-    bool ns = truth(sh, bits);
+    if (!(tnext <= tmax)) return h;
+    bits ^= mask;
+    const bool ns = truth(sh, bits);
     if (ns != state) {
-      float qx = px + tnext * dx, qy = py + tnext * dy, qz = pz + tnext * dz;
+      const float qx = px + tnext * dx, qy = py + tnext * dy, qz = pz + tnext * dz;
+      const int first = __ffs(mask) - 1;
+      const uint32_t rest = mask & (mask - 1u);
+      const int f1 = l_tin[first][tid] == tnext ? l_fin[first][tid] : l_fout[first][tid];
       float nx, ny, nz;
-      prim_normal(S, S.prims[sh.first_prim + first], firstFace, qx, qy, qz, nx, ny, nz);
-      if (second >= 0) {
-        /* simultaneous crossings: mean of the two normals
+      prim_normal(S, S.prims[sh.first_prim + first], f1, qx, qy, qz, nx, ny, nz);
+      if (rest) {
+        /* simultaneous crossings: mean of the two lowest-index normals
          * (NormalIntersectionShape.cu:229-236) */
+        const int second = __ffs(rest) - 1;
+        const int f2 = l_tin[second][tid] == tnext ? l_fin[second][tid] : l_fout[second][tid];
         float mx, my, mz;
-        prim_normal(S, S.prims[sh.first_prim + second], secondFace, qx, qy, qz, mx, my, mz);
+        prim_normal(S, S.prims[sh.first_prim + second], f2, qx, qy, qz, mx, my, mz);
         nx = 0.5f * (nx + mx);
         ny = 0.5f * (ny + my);
         nz = 0.5f * (nz + mz);
@@ -1001,23 +1084,24 @@ MD StepEnd find_end_of_step(const DevScene& S, int r, float px, float py, float 
                             float s) {
   StepEnd se;
   const DRegion& rg = S.regions[r];
-  ShapeHit own = shape_first(S, rg.shape, px, py, pz, dx, dy, dz, s);
-  se.ownNx = own.nx;
-  se.ownNy = own.ny;
-  se.ownNz = own.nz;
-  float t = own.t;
-  int base = (t <= s && rg.parent >= 0) ? rg.parent : r;
+  float t = kInf;
+  int base = r;
   se.winner = -1;
-  se.winNx = own.nx;
-  se.winNy = own.ny;
-  se.winNz = own.nz;
-  for (int c = 0; c < rg.n_children; ++c) {
-    int ch = S.children[rg.first_child + c];
-    ShapeHit h = shape_first(S, S.regions[ch].shape, px, py, pz, dx, dy, dz, fminf(s, t));
-    if (h.t <= s && h.t < t) {
+  /* c = -1: the region's own shape; c >= 0: its children, each limited to
+   * the nearest crossing found so far */
+  for (int c = -1; c < rg.n_children; ++c) {
+    const int reg = c < 0 ? r : S.children[rg.first_child + c];
+    ShapeHit h = shape_first(S, S.regions[reg].shape, px, py, pz, dx, dy, dz, c < 0 ? s : fminf(s, t));
+    if (c < 0) {
+      se.ownNx = se.winNx = h.nx;
+      se.ownNy = se.winNy = h.ny;
+      se.ownNz = se.winNz = h.nz;
       t = h.t;
-      base = ch;
-      se.winner = ch;
+      base = (t <= s && rg.parent >= 0) ? rg.parent : r;
+    } else if (h.t <= s && h.t < t) {
+      t = h.t;
+      base = reg;
+      se.winner = reg;
       se.winNx = h.nx;
       se.winNy = h.ny;
       se.winNz = h.nz;

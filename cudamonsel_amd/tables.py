@@ -65,9 +65,11 @@ def synthetic_inelastic_tables(directory, e_fermi_eV=12.44, gap_eV=1.1, name="sy
     return paths
 
 
-def polynomial_tables(directory, seed=0):
+def polynomial_tables(directory, seed=0, scale0=1.0):
     """Tables whose values are cubic polynomials on non-uniform grids (for
-    interpolation-exactness tests): returns (paths, coefficient dict)."""
+    interpolation-exactness tests): returns (paths, coefficient dict).  The
+    first axis is stored multiplied by scale0 (e.g. an energy grid in J), the
+    values are those of the unscaled polynomial."""
     rng = np.random.default_rng(seed)
     os.makedirs(directory, exist_ok=True)
     g0 = np.sort(rng.uniform(1.0, 9.0, 12))
@@ -87,8 +89,9 @@ def polynomial_tables(directory, seed=0):
     X, Y, Z = np.meshgrid(g0, g1, g2, indexing="ij")
     X2, Y2 = np.meshgrid(g0, g1, indexing="ij")
     paths = [os.path.join(directory, f"poly_{k}.csv") for k in ("1d", "2d", "3d", "2dse")]
-    write_nu_table(paths[0], [g0], p1(g0))
-    write_nu_table(paths[1], [g0, g1], p2(X2, Y2))
-    write_nu_table(paths[2], [g0, g1, g2], p3(X, Y, Z))
-    write_nu_table(paths[3], [g0, g1], p2(X2, Y2))
-    return paths, dict(g0=g0, g1=g1, g2=g2, p1=p1, p2=p2, p3=p3)
+    s0 = g0 * scale0
+    write_nu_table(paths[0], [s0], p1(g0))
+    write_nu_table(paths[1], [s0, g1], p2(X2, Y2))
+    write_nu_table(paths[2], [s0, g1, g2], p3(X, Y, Z))
+    write_nu_table(paths[3], [s0, g1], p2(X2, Y2))
+    return paths, dict(g0=s0, g1=g1, g2=g2, p1=p1, p2=p2, p3=p3, scale0=scale0)

@@ -265,7 +265,7 @@ typedef struct monsel_job_stats {
 
 /* Launch shape; zero fields select defaults. */
 typedef struct monsel_tuning {
-  int32_t block_size;        /* threads per workgroup (multiple of 64) */
+  int32_t block_size;        /* threads per workgroup (multiple of 64, <= 256) */
   int32_t blocks_per_cu;     /* persistent workgroups per CU */
   int32_t max_generations;   /* secondary generations per job */
   int32_t queue_capacity;    /* electrons per generation buffer (0 = sized from the job) */

@@ -94,10 +94,13 @@ def test_find_end_of_step(oracle, compat):
     assert agree / tot >= 0.995
 
 
-def test_nu_tables(oracle, tmp_path, M):
+@pytest.mark.parametrize("scale0", [1.0, 1.602e-16], ids=["unit", "joule"])
+def test_nu_tables(oracle, tmp_path, M, scale0):
+    """Tabulated-inelastic lookups; the joule case puts the first axis in the
+    units the inelastic tables use (energies ~1e-16 J)."""
     from cudamonsel_amd import scenes, tables
     from cudamonsel_amd.abi import probe_for
-    paths, P = tables.polynomial_tables(str(tmp_path))
+    paths, P = tables.polynomial_tables(str(tmp_path), scale0=scale0)
     for compat in (True, False):
         sc = scenes.Scene()
         ch = scenes._chamber(sc)
