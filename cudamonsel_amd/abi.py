@@ -45,7 +45,8 @@ SYMBOLS = [
     "monsel_scene_footprint", "monsel_scene_upload", "monsel_scene_release", "monsel_job_footprint",
     "monsel_job_submit", "monsel_job_poll", "monsel_job_wait", "monsel_job_read", "monsel_job_stats_get",
     "monsel_job_release", "monsel_run", "monsel_probe_rates", "monsel_probe_elastic_angle", "monsel_probe_csd",
-    "monsel_probe_find_end_of_step", "monsel_probe_locate", "monsel_probe_nu_table",
+    "monsel_probe_find_end_of_step", "monsel_probe_locate", "monsel_probe_nu_table", "monsel_probe_scatter",
+    "monsel_probe_barrier",
 ]
 
 
@@ -111,6 +112,31 @@ class Probe:
         _check(self.L.monsel_probe_nu_table(self.ctx, self.scene, int(table), int(order), xx.shape[0], _fp(xx), _fp(out)),
                "probe_nu_table")
         return out
+
+    def scatter(self, msm, mech, prev_eV, E_eV, dirs, u):
+        """Same layout as the oracle's scatter probe (dict of arrays)."""
+        pe = np.ascontiguousarray(prev_eV, dtype=np.float32)
+        e = np.ascontiguousarray(E_eV, dtype=np.float32)
+        d = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)
+        uu = np.ascontiguousarray(u, dtype=np.float32).reshape(-1, 6)
+        n = e.size
+        out = np.zeros((n, 11), np.float32)
+        _check(self.L.monsel_probe_scatter(self.ctx, self.scene, int(msm), int(mech), n, _fp(pe), _fp(e), _fp(d),
+                                           _fp(uu), _fp(out)), "probe_scatter")
+        return dict(dir=out[:, 0:3], E=out[:, 3], has=out[:, 4] > 0, sE=out[:, 5], sdir=out[:, 6:9],
+                    trapped=out[:, 9] > 0)
+
+    def barrier(self, region_from, region_to, dirs, normals, E_eV, u):
+        """Same layout as the oracle's barrier probe (dict of arrays)."""
+        d = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)
+        nb = np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3)
+        e = np.ascontiguousarray(E_eV, dtype=np.float32)
+        uu = np.ascontiguousarray(u, dtype=np.float32)
+        n = e.size
+        out = np.zeros((n, 6), np.float32)
+        _check(self.L.monsel_probe_barrier(self.ctx, self.scene, int(region_from), int(region_to), n, _fp(d), _fp(nb),
+                                           _fp(e), _fp(uu), _fp(out)), "probe_barrier")
+        return dict(dir=out[:, 0:3], E=out[:, 3], region=out[:, 4].astype(int), sign=out[:, 5].astype(int))
 
 
 def probe_for(scene_obj, device=0):

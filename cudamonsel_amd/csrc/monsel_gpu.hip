@@ -183,6 +183,80 @@ __device__ __forceinline__ void tally_escape(const DevScene& S, const JobDev& J,
 
 enum StepResult { STEP_CONTINUE = 0, STEP_DEAD = 1, STEP_ESCAPED = 2, STEP_WATCHDOG = 3 };
 
+/* ExpQMBarrierSM.cu:98-219: the crossing from the electron's region into
+ * `next`, given the outward barrier normal (nbx, nby, nbz) resolved from the
+ * struck shape.  Moves the electron a nudge along +nb (returns +1) or -nb
+ * (returns -1); u1 decides a quantum-mechanical transmission. */
+__device__ __forceinline__ int barrier_core(const DevScene& S, const DMSM& msm, const DMSM& nmsm, Elec& e, int next,
+                                            float nbx, float nby, float nbz, float u1) {
+  const float deltaU = -msm.cb + nmsm.cb;
+  const bool compat = (S.compat & MONSEL_COMPAT_BARRIER_NORMAL) != 0;
+  if (compat) {
+    nbx = e.dx; nby = e.dy; nbz = e.dz; /* ExpQMBarrierSM.cu:111 */
+  } else {
+    float m2 = nbx * nbx + nby * nby + nbz * nbz;
+    if (m2 > 0.f) {
+      float inv = __builtin_amdgcn_rsqf(m2);
+      nbx *= inv; nby *= inv; nbz *= inv;
+    } else {
+      nbx = e.dx; nby = e.dy; nbz = e.dz;
+    }
+  }
+  const float cosalpha = compat ? 1.f : (e.dx * nbx + e.dy * nby + e.dz * nbz);
+  const float dd = nudge_nm(e.px, e.py, e.pz);
+  if (cosalpha <= 0.f) {
+    e.px -= dd * nbx; e.py -= dd * nby; e.pz -= dd * nbz;
+    return -1;
+  }
+  if (deltaU == 0.f) {
+    e.px += dd * nbx; e.py += dd * nby; e.pz += dd * nbz;
+    e.region = next;
+    return 1;
+  }
+  const float kE0 = e.E;
+  const float perpE = kE0 <= 0.f ? 0.f : cosalpha * cosalpha * kE0;
+  bool transmits;
+  float rootPerpE = 0.f, rootDiff = 0.f;
+  if (perpE == 0.f || perpE <= deltaU) transmits = false;
+  else {
+    rootPerpE = __builtin_sqrtf(perpE);
+    rootDiff = __builtin_sqrtf(perpE - deltaU);
+    if (msm.barrier_kind == MONSEL_BARRIER_EXPQM_CLASSICAL) transmits = true;
+    else {
+      float tp;
+      if (msm.lambda == 0.f) { /* ExpQMBarrierSM.cu:30-34 */
+        float ks = rootPerpE + rootDiff;
+        tp = (4.f * rootPerpE * rootDiff) / (ks * ks);
+      } else { /* ExpQMBarrierSM.cu:36-48 */
+        float k1 = msm.lambdaFactor * rootPerpE;
+        if (k1 > 50.f) tp = 1.f;
+        else {
+          float k2 = msm.lambdaFactor * rootDiff;
+          float ratio = sinhf(k1 - k2) / sinhf(k1 + k2);
+          tp = 1.f - ratio * ratio;
+        }
+      }
+      transmits = u1 < tp;
+    }
+  }
+  if (transmits) {
+    float factor = cosalpha * ((rootDiff / rootPerpE) - 1.f);
+    float nx = e.dx + factor * nbx, ny = e.dy + factor * nby, nz = e.dz + factor * nbz;
+    float inv = __builtin_amdgcn_rsqf(nx * nx + ny * ny + nz * nz);
+    e.dx = nx * inv; e.dy = ny * inv; e.dz = nz * inv;
+    e.E = kE0 - deltaU;
+    e.region = next;
+    e.px += dd * nbx; e.py += dd * nby; e.pz += dd * nbz;
+    return 1;
+  }
+  float two = 2.f * cosalpha;
+  float nx = e.dx - two * nbx, ny = e.dy - two * nby, nz = e.dz - two * nbz;
+  float inv = __builtin_amdgcn_rsqf(nx * nx + ny * ny + nz * nz);
+  e.dx = nx * inv; e.dy = ny * inv; e.dz = nz * inv;
+  e.px -= dd * nbx; e.py -= dd * nby; e.pz -= dd * nbz;
+  return -1;
+}
+
 /* One MonteCarloSS::takeStep (MonteCarloSS.cu:107-163). */
 __device__ __forceinline__ int take_step(const DevScene& S, const JobDev& J, Elec& e, Spawn& sp) {
   sp.has = false;
@@ -257,12 +331,10 @@ __device__ __forceinline__ int take_step(const DevScene& S, const JobDev& J, Ele
     if (msm.is_null || msm.barrier_kind == MONSEL_BARRIER_NONE) {
       e.region = next;
     } else {
-      const DMSM& nmsm = S.msms[S.regions[next].msm];
-      float deltaU = -msm.cb + nmsm.cb;
-      float nbx, nby, nbz;
-      if (S.compat & MONSEL_COMPAT_BARRIER_NORMAL) {
-        nbx = e.dx; nby = e.dy; nbz = e.dz; /* ExpQMBarrierSM.cu:111 */
-      } else {
+      /* ExpQMBarrierSM.cu:77-96: outward normal of the struck shape, negated
+       * when entering a child region (unused in compat mode) */
+      float nbx = 0.f, nby = 0.f, nbz = 0.f;
+      if (!(S.compat & MONSEL_COMPAT_BARRIER_NORMAL)) {
         if (is_containing_region(S, region, next)) {
           int struck = next;
           while (S.regions[struck].parent != region) struck = S.regions[struck].parent;
@@ -275,64 +347,8 @@ __device__ __forceinline__ int take_step(const DevScene& S, const JobDev& J, Ele
         } else {
           nbx = se.ownNx; nby = se.ownNy; nbz = se.ownNz;
         }
-        float m2 = nbx * nbx + nby * nby + nbz * nbz;
-        if (m2 > 0.f) {
-          float inv = __builtin_amdgcn_rsqf(m2);
-          nbx *= inv; nby *= inv; nbz *= inv;
-        } else {
-          nbx = e.dx; nby = e.dy; nbz = e.dz;
-        }
       }
-      float cosalpha = (S.compat & MONSEL_COMPAT_BARRIER_NORMAL) ? 1.f : (e.dx * nbx + e.dy * nby + e.dz * nbz);
-      float dd = nudge_nm(e.px, e.py, e.pz);
-      if (cosalpha <= 0.f) {
-        e.px -= dd * nbx; e.py -= dd * nby; e.pz -= dd * nbz;
-      } else if (deltaU == 0.f) {
-        e.px += dd * nbx; e.py += dd * nby; e.pz += dd * nbz;
-        e.region = next;
-      } else {
-        float kE0 = e.E;
-        float perpE = kE0 <= 0.f ? 0.f : cosalpha * cosalpha * kE0;
-        bool transmits;
-        float rootPerpE = 0.f, rootDiff = 0.f;
-        if (perpE == 0.f || perpE <= deltaU) transmits = false;
-        else {
-          rootPerpE = __builtin_sqrtf(perpE);
-          rootDiff = __builtin_sqrtf(perpE - deltaU);
-          if (msm.barrier_kind == MONSEL_BARRIER_EXPQM_CLASSICAL) transmits = true;
-          else {
-            float tp;
-            if (msm.lambda == 0.f) {
-              float ks = rootPerpE + rootDiff;
-              tp = (4.f * rootPerpE * rootDiff) / (ks * ks);
-            } else {
-              float k1 = msm.lambdaFactor * rootPerpE;
-              if (k1 > 50.f) tp = 1.f;
-              else {
-                float k2 = msm.lambdaFactor * rootDiff;
-                float ratio = sinhf(k1 - k2) / sinhf(k1 + k2);
-                tp = 1.f - ratio * ratio;
-              }
-            }
-            transmits = u[1] < tp;
-          }
-        }
-        if (transmits) {
-          float factor = cosalpha * ((rootDiff / rootPerpE) - 1.f);
-          float nx = e.dx + factor * nbx, ny = e.dy + factor * nby, nz = e.dz + factor * nbz;
-          float inv = __builtin_amdgcn_rsqf(nx * nx + ny * ny + nz * nz);
-          e.dx = nx * inv; e.dy = ny * inv; e.dz = nz * inv;
-          e.E = kE0 - deltaU;
-          e.region = next;
-          e.px += dd * nbx; e.py += dd * nby; e.pz += dd * nbz;
-        } else {
-          float two = 2.f * cosalpha;
-          float nx = e.dx - two * nbx, ny = e.dy - two * nby, nz = e.dz - two * nbz;
-          float inv = __builtin_amdgcn_rsqf(nx * nx + ny * ny + nz * nz);
-          e.dx = nx * inv; e.dy = ny * inv; e.dz = nz * inv;
-          e.px -= dd * nbx; e.py -= dd * nby; e.pz -= dd * nbz;
-        }
-      }
+      barrier_core(S, msm, S.msms[S.regions[next].msm], e, next, nbx, nby, nbz, u[1]);
     }
     /* MonteCarloSS.cu:146-148: nudge along the new direction; the region is
      * re-located by the check that opens the next step */
@@ -595,6 +611,65 @@ __global__ void k_probe_elastic(DevScene S, int mech, int element, int n, const 
   else
     ca = mott_cos(S, el.mott, EeV, r1);
   out[i] = acosf(fminf(1.f, fmaxf(-1.f, ca)));
+}
+
+/* One mechanism scatter with injected uniforms u[6i..6i+5] (slots 2..7 of
+ * the step's uniforms); out[11i..]: dir (3), E, secondary flag, secondary
+ * E, secondary dir (3), trapped flag, 0. */
+__global__ void k_probe_scatter(DevScene S, int mech, int n, const float* prevE, const float* E, const float* dir,
+                                const float* u, float* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const DMech& m = S.mechs[mech];
+  Elec e;
+  e.px = e.py = e.pz = 0.f;
+  e.dx = dir[3 * i];
+  e.dy = dir[3 * i + 1];
+  e.dz = dir[3 * i + 2];
+  e.E = E[i];
+  float uu[kStepUniforms];
+  uu[0] = uu[1] = 0.5f;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) uu[2 + k] = u[6 * i + k];
+  bool trapped = false;
+  const float rate = mech_rate(S, m, prevE[i]);
+  Spawn sp = mech_scatter(S, m, e, prevE[i], rate, trapped, uu);
+  float* o = out + 11 * (size_t)i;
+  o[0] = e.dx;
+  o[1] = e.dy;
+  o[2] = e.dz;
+  o[3] = e.E;
+  o[4] = sp.has ? 1.f : 0.f;
+  o[5] = sp.has ? sp.E : 0.f;
+  o[6] = sp.has ? sp.dx : 0.f;
+  o[7] = sp.has ? sp.dy : 0.f;
+  o[8] = sp.has ? sp.dz : 0.f;
+  o[9] = trapped ? 1.f : 0.f;
+  o[10] = 0.f;
+}
+
+/* One barrier crossing from region `from` into `to` with the outward barrier
+ * normal given; out[6i..]: dir (3), E, region after, nudge sign. */
+__global__ void k_probe_barrier(DevScene S, int from, int to, int n, const float* dir, const float* nb, const float* E,
+                                const float* u, float* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Elec e;
+  e.px = e.py = e.pz = 0.f;
+  e.dx = dir[3 * i];
+  e.dy = dir[3 * i + 1];
+  e.dz = dir[3 * i + 2];
+  e.E = E[i];
+  e.region = from;
+  int sign = barrier_core(S, S.msms[S.regions[from].msm], S.msms[S.regions[to].msm], e, to, nb[3 * i], nb[3 * i + 1],
+                          nb[3 * i + 2], u[i]);
+  float* o = out + 6 * (size_t)i;
+  o[0] = e.dx;
+  o[1] = e.dy;
+  o[2] = e.dz;
+  o[3] = e.E;
+  o[4] = (float)e.region;
+  o[5] = (float)sign;
 }
 
 __global__ void k_probe_csd(DevScene S, int msm, int n, const float* len, const float* E, float* out) {
@@ -1771,6 +1846,60 @@ int monsel_probe_nu_table(monsel_ctx ctx, monsel_scene_id scene, int table, int 
     hipLaunchKernelGGL(k_probe_nu, dim3(grid_for(n)), dim3(256), 0, 0, sd->dev, table, order, n, dX.d, dO.d);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpy(out, dO.d, 4 * (size_t)n, hipMemcpyDeviceToHost));
+    return MONSEL_OK;
+  });
+}
+
+int monsel_probe_scatter(monsel_ctx ctx, monsel_scene_id scene, int msm, int mech, int n, const float* prev_ev,
+                         const float* energies_ev, const float* dir, const float* u, float* out) {
+  std::shared_ptr<SceneDev> sd;
+  int st = probe_prologue(ctx, scene, sd);
+  if (st) return st;
+  if (msm < 0 || msm >= sd->dev.n_msm || n < 0 || (n > 0 && (!prev_ev || !energies_ev || !dir || !u || !out)))
+    return set_err(MONSEL_E_INVALID, "probe_scatter arguments");
+  if (n == 0) return MONSEL_OK;
+  return guarded([&] {
+    DMSM m;
+    HIP_TRY(hipMemcpy(&m, sd->dev.msms + msm, sizeof(DMSM), hipMemcpyDeviceToHost));
+    if (m.is_null || mech < 0 || mech >= m.n_mech) throw MonselError(MONSEL_E_INVALID, "mechanism slot out of range");
+    Scratch<float> dP(n), dE(n), dD(3 * (size_t)n), dU(6 * (size_t)n), dO(11 * (size_t)n);
+    HIP_TRY(hipMemcpy(dP.d, prev_ev, 4 * (size_t)n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dE.d, energies_ev, 4 * (size_t)n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dD.d, dir, 12 * (size_t)n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dU.d, u, 24 * (size_t)n, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_probe_scatter, dim3(grid_for(n)), dim3(256), 0, 0, sd->dev, m.mech0 + mech, n, dP.d, dE.d,
+                       dD.d, dU.d, dO.d);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(out, dO.d, 44 * (size_t)n, hipMemcpyDeviceToHost));
+    return MONSEL_OK;
+  });
+}
+
+int monsel_probe_barrier(monsel_ctx ctx, monsel_scene_id scene, int region_from, int region_to, int n,
+                         const float* dir, const float* normal, const float* energies_ev, const float* u, float* out) {
+  std::shared_ptr<SceneDev> sd;
+  int st = probe_prologue(ctx, scene, sd);
+  if (st) return st;
+  if (region_from < 0 || region_from >= sd->dev.n_regions || region_to < 0 || region_to >= sd->dev.n_regions ||
+      region_from == region_to || n < 0 || (n > 0 && (!dir || !normal || !energies_ev || !u || !out)))
+    return set_err(MONSEL_E_INVALID, "probe_barrier arguments");
+  if (n == 0) return MONSEL_OK;
+  return guarded([&] {
+    DRegion rf;
+    HIP_TRY(hipMemcpy(&rf, sd->dev.regions + region_from, sizeof(DRegion), hipMemcpyDeviceToHost));
+    DMSM m;
+    HIP_TRY(hipMemcpy(&m, sd->dev.msms + rf.msm, sizeof(DMSM), hipMemcpyDeviceToHost));
+    if (m.is_null || m.barrier_kind == MONSEL_BARRIER_NONE)
+      throw MonselError(MONSEL_E_INVALID, "region has no barrier mechanism");
+    Scratch<float> dD(3 * (size_t)n), dN(3 * (size_t)n), dE(n), dU(n), dO(6 * (size_t)n);
+    HIP_TRY(hipMemcpy(dD.d, dir, 12 * (size_t)n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dN.d, normal, 12 * (size_t)n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dE.d, energies_ev, 4 * (size_t)n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dU.d, u, 4 * (size_t)n, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_probe_barrier, dim3(grid_for(n)), dim3(256), 0, 0, sd->dev, region_from, region_to, n, dD.d,
+                       dN.d, dE.d, dU.d, dO.d);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(out, dO.d, 24 * (size_t)n, hipMemcpyDeviceToHost));
     return MONSEL_OK;
   });
 }

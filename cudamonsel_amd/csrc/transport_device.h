@@ -89,9 +89,9 @@ struct Elec {
 MD float u01(uint32_t x) { return (float)(x >> 9) * 1.1920928955078125e-07f + 5.9604644775390625e-08f; }
 
 /* Uniforms per electron step: Philox blocks 2s and 2s+1 of the electron's
- * stream, s = its step count.  Every draw of a step has a fixed slot:
- *   u[0] free path, u[1] mechanism choice or barrier transmission,
- *   u[2..7] the scatter mechanism's draws (at most six). */
+ * stream, s = its step count.  u[0] draws the free path, u[1] the mechanism
+ * (or a barrier transmission); the scatter mechanism takes u[2], u[3], ...
+ * in the order the reference's scatter() draws them (at most six). */
 constexpr int kStepUniforms = 8;
 
 MD void step_uniforms(const Elec& e, uint32_t k0, uint32_t k1, float* u) {
@@ -722,19 +722,24 @@ MD Spawn mech_scatter(const DevScene& S, const DMech& m, Elec& e, float prevE, f
         cum += elem_sigma(S, el, prevE) * el.natoms;
       }
       const DElem& el = S.elems[m.elem0 + idx];
-      float ca;
+      /* the polar angle takes one uniform (two for Browning), the azimuth
+       * the next one (SelectableElasticSM.cu:62-64) */
+      float ca, ub;
       if (el.kind == MONSEL_MECH_BROWNING_ELASTIC ||
           (el.kind == MONSEL_MECH_NISTMOTT_ELASTIC && prevE < el.extrap)) {
         ca = browning_cos(el, prevE, u[3], u[4]);
+        ub = u[5];
       } else if (el.kind == MONSEL_MECH_RUTHERFORD_ELASTIC || prevE >= kMaxNISTMott) {
         ca = rutherford_cos(el, prevE, u[3]);
+        ub = u[4];
       } else {
         ca = mott_cos(S, el.mott, prevE, u[3]);
+        ub = u[4];
       }
       ca = fminf(1.f, fmaxf(-1.f, ca));
       float sa = __builtin_sqrtf(fmaxf(0.f, 1.f - ca * ca));
       float sb, cb;
-      __sincosf(kTwoPI * u[5], &sb, &cb);
+      __sincosf(kTwoPI * ub, &sb, &cb);
       rotate_dir(e.dx, e.dy, e.dz, ca, sa, cb, sb);
       return se;
     }

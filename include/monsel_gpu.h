@@ -338,6 +338,27 @@ int monsel_probe_locate(monsel_ctx ctx, monsel_scene_id scene, int n, const floa
 int monsel_probe_nu_table(monsel_ctx ctx, monsel_scene_id scene, int table, int order, int n,
                           const float* x, float* out);
 
+/* ScatterMechanism::scatter of mechanism slot `mech` of MSM `msm`
+ * (SelectableElasticSM.cu:49-67, FittedInelSM.cu:13-18,
+ * GanachaudMokraniPolaronTrapSM.cu:15-20, GanachaudMokraniPhononInelasticSM.cu:19-44,
+ * TabulatedInelasticSM.cu:209-434) for an electron at energies_ev[i] after a
+ * step from prev_ev[i] (the elastic choice uses the pre-step energy), unit
+ * direction dir[3*i], with the uniforms u[6*i .. 6*i+5] taken in the order the
+ * reference's scatter() draws them.  out[11*i ..]: direction (3), energy (eV),
+ * secondary flag, secondary energy (eV), secondary direction (3), trapped flag,
+ * 0. */
+int monsel_probe_scatter(monsel_ctx ctx, monsel_scene_id scene, int msm, int mech, int n, const float* prev_ev,
+                         const float* energies_ev, const float* dir, const float* u, float* out);
+
+/* ExpQMBarrierSM::barrierScatter (ExpQMBarrierSM.cu:98-219) from region
+ * `region_from` into `region_to` for unit direction dir[3*i], outward barrier
+ * normal normal[3*i] (as resolved from the struck shape by
+ * ExpQMBarrierSM.cu:77-96), energy energies_ev[i] and transmission uniform
+ * u[i].  out[6*i ..]: direction (3), energy (eV), region after, nudge sign
+ * (+1 along the normal, -1 against it). */
+int monsel_probe_barrier(monsel_ctx ctx, monsel_scene_id scene, int region_from, int region_to, int n,
+                         const float* dir, const float* normal, const float* energies_ev, const float* u, float* out);
+
 #ifdef __cplusplus
 }
 #endif

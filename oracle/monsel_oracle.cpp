@@ -1289,28 +1289,15 @@ static double general_barrier_T(double lambdaFactor, double rootPerpE, double ro
   return 1. - (ratio * ratio);
 }
 
-/* ExpQMBarrierSM.cu:50-219 (and NullMaterialScatterModel.cu:14-20). */
-static void barrier_scatter(Ctx& cx, const MSM& msm, Electron& pe, int nextRegion) {
-  const Scene& sc = cx.sc;
-  if (msm.isNull || msm.barrierKind == MONSEL_BARRIER_NONE) {
-    pe.setRegion(nextRegion);
-    return;
-  }
+/* ExpQMBarrierSM.cu:98-219: the crossing itself, given the outward barrier
+ * normal nb (ExpQMBarrierSM.cu:77-96 resolves it from the struck shape).
+ * Returns +1 when the electron moved (or was nudged) along +nb, -1 along -nb. */
+static int barrier_core(const Scene& sc, const MSM& msm, Electron& pe, int nextRegion, const double nbIn[3]) {
   int cur = pe.region;
-  if (cur == nextRegion) return;
   const Material& curMat = sc.materials[sc.msms[sc.regions[cur].msm].material];
   const Material& nextMat = sc.materials[sc.msms[sc.regions[nextRegion].msm].material];
   double deltaU = -curMat.cb + nextMat.cb;
-  double nb[3] = {0, 0, 0};
-  if (cx.geo.is_containing_region(cur, nextRegion)) {
-    int struck = nextRegion;
-    while (sc.regions[struck].parent != cur) struck = sc.regions[struck].parent;
-    const NV& v = cx.shapeNormal[sc.regions[struck].shape];
-    for (int i = 0; i < 3; i++) nb[i] = -v.n[i];
-  } else {
-    const NV& v = cx.shapeNormal[sc.regions[cur].shape];
-    for (int i = 0; i < 3; i++) nb[i] = v.n[i];
-  }
+  double nb[3] = {nbIn[0], nbIn[1], nbIn[2]};
   double st0 = ::sin(pe.theta);
   double n0[3] = {st0 * ::cos(pe.phi), st0 * ::sin(pe.phi), ::cos(pe.theta)};
   if (sc.barrierCompat()) {
@@ -1323,12 +1310,12 @@ static void barrier_scatter(Ctx& cx, const MSM& msm, Electron& pe, int nextRegio
   double cosalpha = n0[0] * nb[0] + n0[1] * nb[1] + n0[2] * nb[2];
   if (cosalpha <= 0.) {
     for (int i = 0; i < 3; ++i) pe.pos[i] -= SMALL_DISP * nb[i];
-    return;
+    return -1;
   }
   if (deltaU == 0.) {
     for (int i = 0; i < 3; ++i) pe.pos[i] += SMALL_DISP * nb[i];
     pe.setRegion(nextRegion);
-    return;
+    return 1;
   }
   double kE0 = pe.E;
   double perpE = kE0 <= 0. ? 0. : cosalpha * cosalpha * kE0;
@@ -1360,6 +1347,31 @@ static void barrier_scatter(Ctx& cx, const MSM& msm, Electron& pe, int nextRegio
   }
   pe.theta = ::acos(nf[2]);
   pe.phi = ::atan2(nf[1], nf[0]);
+  return transmits ? 1 : -1;
+}
+
+/* ExpQMBarrierSM.cu:50-219 (and NullMaterialScatterModel.cu:14-20). */
+static void barrier_scatter(Ctx& cx, const MSM& msm, Electron& pe, int nextRegion) {
+  const Scene& sc = cx.sc;
+  if (msm.isNull || msm.barrierKind == MONSEL_BARRIER_NONE) {
+    pe.setRegion(nextRegion);
+    return;
+  }
+  int cur = pe.region;
+  if (cur == nextRegion) return;
+  /* ExpQMBarrierSM.cu:77-96: outward normal of the struck shape, negated
+   * when the electron enters a child region */
+  double nb[3] = {0, 0, 0};
+  if (cx.geo.is_containing_region(cur, nextRegion)) {
+    int struck = nextRegion;
+    while (sc.regions[struck].parent != cur) struck = sc.regions[struck].parent;
+    const NV& v = cx.shapeNormal[sc.regions[struck].shape];
+    for (int i = 0; i < 3; i++) nb[i] = -v.n[i];
+  } else {
+    const NV& v = cx.shapeNormal[sc.regions[cur].shape];
+    for (int i = 0; i < 3; i++) nb[i] = v.n[i];
+  }
+  barrier_core(sc, msm, pe, nextRegion, nb);
 }
 
 /* MONSEL_MaterialScatterModel.cu:44-62: cumulative mechanism rates. */
@@ -1920,6 +1932,83 @@ int oracle_probe_find_end_of_step(void* scene, int region, int n, const double* 
       for (int k = 0; k < 3; ++k) nrm[k] = win->n[k];
     }
     for (int k = 0; k < 3; ++k) out_normal[3 * i + k] = nrm[k];
+  }
+  return 0;
+}
+
+static void dir_to_angles(const double* d, double* theta, double* phi) {
+  *theta = ::atan2(::sqrt(d[0] * d[0] + d[1] * d[1]), d[2]);
+  *phi = ::atan2(d[1], d[0]);
+}
+static void angles_to_dir(double theta, double phi, double* d) {
+  double st = ::sin(theta);
+  d[0] = st * ::cos(phi);
+  d[1] = st * ::sin(phi);
+  d[2] = ::cos(theta);
+}
+
+/* One ScatterMechanism::scatter of mechanism `mech` of MSM `msm` with the
+ * electron at energy e_ev after a step from energy prev_ev, direction dir,
+ * and the uniforms u[6i..6i+5] consumed in order.  Per sample, out[11i..]:
+ * direction (3), energy (eV), secondary flag, secondary energy (eV),
+ * secondary direction (3), trapped flag, uniforms consumed. */
+int oracle_probe_scatter(void* scene, int msm, int mech, int n, const double* prev_ev, const double* e_ev,
+                         const double* dir, const double* u, double* out) {
+  const Scene& sc = *(const Scene*)scene;
+  if (msm < 0 || msm >= (int)sc.msms.size()) return MONSEL_E_INVALID;
+  const MSM& m = sc.msms[msm];
+  if (mech < 0 || mech >= (int)m.mech.size()) return MONSEL_E_INVALID;
+  for (int i = 0; i < n; ++i) {
+    Electron pe;
+    double p0[3] = {0, 0, 0}, th, ph;
+    dir_to_angles(dir + 3 * i, &th, &ph);
+    pe.init(p0, th, ph, e_ev[i] * TOSI_EV);
+    pe.prevE = prev_ev[i] * TOSI_EV;
+    pe.rng.script = u + 6 * (size_t)i;
+    Spawn sp = mech_scatter(sc, m.mech[mech], pe);
+    double* o = out + 11 * (size_t)i;
+    angles_to_dir(pe.theta, pe.phi, o);
+    o[3] = pe.E * FROMSI_EV;
+    o[4] = sp.has ? 1. : 0.;
+    o[5] = sp.has ? sp.E * FROMSI_EV : 0.;
+    double sd[3] = {0, 0, 0};
+    if (sp.has) {
+      Electron se;
+      se.init(p0, sp.theta, sp.phi, sp.E);
+      if (sp.rotate) se.updateDirection(sp.dTheta, sp.dPhi);
+      angles_to_dir(se.theta, se.phi, sd);
+    }
+    for (int k = 0; k < 3; ++k) o[6 + k] = sd[k];
+    o[9] = pe.complete ? 1. : 0.;
+    o[10] = (double)pe.rng.draws;
+  }
+  return 0;
+}
+
+/* One barrier crossing from region `from` into region `to` with the
+ * outward barrier normal nb[3i..] already resolved (ExpQMBarrierSM.cu:77-96),
+ * direction dir, energy e_ev and the transmission uniform u[i].  Per sample,
+ * out[6i..]: direction (3), energy (eV), region after, nudge sign (+1 along
+ * nb, -1 against). */
+int oracle_probe_barrier(void* scene, int from, int to, int n, const double* dir, const double* nb,
+                         const double* e_ev, const double* u, double* out) {
+  const Scene& sc = *(const Scene*)scene;
+  if (from < 0 || from >= (int)sc.regions.size() || to < 0 || to >= (int)sc.regions.size()) return MONSEL_E_INVALID;
+  const MSM& m = sc.msms[sc.regions[from].msm];
+  if (m.isNull || m.barrierKind == MONSEL_BARRIER_NONE || from == to) return MONSEL_E_INVALID;
+  for (int i = 0; i < n; ++i) {
+    Electron pe;
+    double p0[3] = {0, 0, 0}, th, ph;
+    dir_to_angles(dir + 3 * i, &th, &ph);
+    pe.init(p0, th, ph, e_ev[i] * TOSI_EV);
+    pe.region = from;
+    pe.rng.script = u + i;
+    int sign = barrier_core(sc, m, pe, to, nb + 3 * (size_t)i);
+    double* o = out + 6 * (size_t)i;
+    angles_to_dir(pe.theta, pe.phi, o);
+    o[3] = pe.E * FROMSI_EV;
+    o[4] = (double)pe.region;
+    o[5] = (double)sign;
   }
   return 0;
 }

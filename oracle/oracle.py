@@ -74,7 +74,7 @@ def lib():
                                      ctypes.POINTER(JobStats), ctypes.c_int]
         for name in ("oracle_probe_rates", "oracle_probe_csd", "oracle_probe_elastic_angle",
                      "oracle_probe_elastic_sigma", "oracle_probe_find_end_of_step", "oracle_probe_locate",
-                     "oracle_probe_nu_table"):
+                     "oracle_probe_nu_table", "oracle_probe_scatter", "oracle_probe_barrier"):
             getattr(L, name).restype = ctypes.c_int
         L.oracle_u_d1.restype = ctypes.c_double
         L.oracle_u_d1.argtypes = [c_double_p, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
@@ -190,6 +190,37 @@ class OracleScene:
         if rc:
             raise ValueError("bad table probe arguments")
         return out
+
+    def scatter(self, msm, mech, prev_eV, E_eV, dirs, u):
+        """One scatter event per row; u is (n, 6), consumed in order.  Returns
+        a dict of arrays: dir (n,3), E, has, sE, sdir (n,3), trapped, draws."""
+        pe = np.ascontiguousarray(prev_eV, dtype=np.float64)
+        e = np.ascontiguousarray(E_eV, dtype=np.float64)
+        d = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+        uu = np.ascontiguousarray(u, dtype=np.float64).reshape(-1, 6)
+        n = e.size
+        out = np.zeros((n, 11))
+        rc = lib().oracle_probe_scatter(ctypes.c_void_p(self.h), int(msm), int(mech), n, dptr(pe), dptr(e), dptr(d),
+                                        dptr(uu), dptr(out))
+        if rc:
+            raise ValueError("bad scatter probe arguments")
+        return dict(dir=out[:, 0:3], E=out[:, 3], has=out[:, 4] > 0, sE=out[:, 5], sdir=out[:, 6:9],
+                    trapped=out[:, 9] > 0, draws=out[:, 10].astype(int))
+
+    def barrier(self, region_from, region_to, dirs, normals, E_eV, u):
+        """One barrier crossing per row with the outward barrier normal given.
+        Returns dict: dir (n,3), E, region, sign (+1 along the normal)."""
+        d = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+        nb = np.ascontiguousarray(normals, dtype=np.float64).reshape(-1, 3)
+        e = np.ascontiguousarray(E_eV, dtype=np.float64)
+        uu = np.ascontiguousarray(u, dtype=np.float64)
+        n = e.size
+        out = np.zeros((n, 6))
+        rc = lib().oracle_probe_barrier(ctypes.c_void_p(self.h), int(region_from), int(region_to), n, dptr(d),
+                                        dptr(nb), dptr(e), dptr(uu), dptr(out))
+        if rc:
+            raise ValueError("bad barrier probe arguments")
+        return dict(dir=out[:, 0:3], E=out[:, 3], region=out[:, 4].astype(int), sign=out[:, 5].astype(int))
 
 
 def philox(ctr, key):

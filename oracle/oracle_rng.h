@@ -91,14 +91,18 @@ struct Stream {
   uint32_t primary = 0;
   uint32_t draws = 0;
   Philox4 buf{};
+  /* probes: when set, next() returns script[0], script[1], ... instead */
+  const double* script = nullptr;
 
   void init(uint64_t s, uint64_t lin, uint32_t prim) {
     seed = s;
     lineage = lin;
     primary = prim;
     draws = 0;
+    script = nullptr;
   }
   double next() {
+    if (script) return script[draws++];
     if (g_rng15 == 2) return msvc_rand01();
     uint32_t slot = draws & 3u;
     if (slot == 0) {

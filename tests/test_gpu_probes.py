@@ -121,3 +121,86 @@ def test_nu_tables(oracle, tmp_path, M, scale0):
             o = osc.nu_table(table, order, x[:, :dim], dim)
             scale = np.max(np.abs(o)) + 1.0
             assert np.max(np.abs(g - o)) / scale < 1e-4
+
+
+def _unit(rng, n):
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    return d.astype(np.float32).astype(np.float64)
+
+
+def _compare_scatter(g, o, rows_ok=0.995, sdir_tol=1e-3):
+    """Per-row agreement of two scatter-probe results; rows whose outcome
+    differs (FP32 vs FP64 at a choice boundary) are allowed up to 1-rows_ok."""
+    same = (g["has"] == o["has"]) & (g["trapped"] == o["trapped"])
+    ddir = np.max(np.abs(g["dir"] - o["dir"]), axis=1)
+    dE = np.abs(g["E"] - o["E"]) <= 1e-5 * np.abs(o["E"]) + 1e-6
+    dsE = np.abs(g["sE"] - o["sE"]) <= 1e-4 * np.abs(o["sE"]) + 1e-5
+    dsd = np.max(np.abs(g["sdir"] - o["sdir"]), axis=1) <= sdir_tol
+    ok = same & (ddir < 2e-4) & dE & dsE & (dsd | ~o["has"])
+    assert ok.mean() >= rows_ok, (ok.mean(), np.max(ddir))
+
+
+@pytest.mark.parametrize("mech", [0, 1, 2], ids=["elastic", "fitted", "polaron"])
+def test_scatter_pmma(pmma, mech):
+    sc, cs, pr, osc = pmma
+    rng = np.random.default_rng(20 + mech)
+    n = 20000
+    E = np.geomspace(2.0, 19000.0, n).astype(np.float32).astype(np.float64)
+    prev = (E * rng.uniform(1.0, 1.05, n)).astype(np.float32).astype(np.float64)
+    d = _unit(rng, n)
+    u = rng.random((n, 6)).astype(np.float32).astype(np.float64)
+    _compare_scatter(pr.scatter(1, mech, prev, E, d, u), osc.scatter(1, mech, prev, E, d, u))
+
+
+def test_scatter_si_phonon(si):
+    sc, cs, pr, osc = si
+    rng = np.random.default_rng(30)
+    n = 20000
+    E = np.geomspace(0.064, 50.0, n).astype(np.float32).astype(np.float64)
+    d = _unit(rng, n)
+    u = rng.random((n, 6)).astype(np.float32).astype(np.float64)
+    _compare_scatter(pr.scatter(1, 2, E, E, d, u), osc.scatter(1, 2, E, E, d, u))
+
+
+@pytest.mark.parametrize("method", [1, 2, 3])
+def test_scatter_tabulated(oracle, M, tmp_path, method):
+    from cudamonsel_amd import scenes, tables
+    from cudamonsel_amd.abi import probe_for
+    paths = tables.synthetic_inelastic_tables(str(tmp_path))
+    sc = scenes.Scene()
+    ch = scenes._chamber(sc)
+    msm = scenes.si_tabulated(sc, paths, method_se=method)
+    shp = scenes._half_space(0.0)
+    reg = M.Region(ch, msm, shp)
+    sc.hold(shp, reg)
+    scenes._finish(sc, 1000.0, 0.5, 0.0)
+    pr, cs = probe_for(sc)
+    osc = oracle.OracleScene(cs)
+    rng = np.random.default_rng(40 + method)
+    n = 20000
+    E = np.geomspace(5.0, 5000.0, n).astype(np.float32).astype(np.float64)
+    d = _unit(rng, n)
+    u = rng.random((n, 6)).astype(np.float32).astype(np.float64)
+    _compare_scatter(pr.scatter(1, 1, E, E, d, u), osc.scatter(1, 1, E, E, d, u), rows_ok=0.99, sdir_tol=2e-3)
+
+
+@pytest.mark.parametrize("compat,lam", [(False, None), (True, None), (False, 0.0), (False, 0.3e-9)],
+                         ids=["fixed", "compat", "sharp-qm", "smooth-qm"])
+def test_barrier(oracle, M, compat, lam):
+    from cudamonsel_amd import scenes
+    from cudamonsel_amd.abi import probe_for
+    sc = scenes.flat("PMMA", beam_eV=500.0, compat=compat, barrier_lambda_m=lam)
+    pr, cs = probe_for(sc)
+    osc = oracle.OracleScene(cs)
+    rng = np.random.default_rng(50)
+    n = 20000
+    d = _unit(rng, n)
+    nb = _unit(rng, n)
+    E = rng.uniform(0.05, 5.0, n).astype(np.float32).astype(np.float64)
+    u = rng.random(n).astype(np.float32).astype(np.float64)
+    g = pr.barrier(1, 0, d, nb, E, u)
+    o = osc.barrier(1, 0, d, nb, E, u)
+    same = (g["region"] == o["region"]) & (g["sign"] == o["sign"])
+    close = (np.max(np.abs(g["dir"] - o["dir"]), axis=1) < 2e-4) & (np.abs(g["E"] - o["E"]) < 1e-5 * E + 1e-6)
+    assert np.mean(same & close) >= 0.999, np.mean(same & close)
