@@ -51,9 +51,11 @@ typedef enum monsel_status {
   MONSEL_E_NOTREADY = -7     /* job still running */
 } monsel_status;
 
-/* Reference-compatibility switches (the "RefCompat" of the design notes).
- * Each bit reproduces one deterministic defect of the reference; clear it to
- * get the corrected behaviour.  MONSEL_COMPAT_REFERENCE sets all of them. */
+/* Reference-compatibility switches ("RefCompat").  Each bit reproduces one
+ * deterministic defect of the reference; clear it to get the corrected
+ * behaviour.  MONSEL_COMPAT_REFERENCE sets all of them.  (The left-corner
+ * defect of NShapes.cu:140 is chosen when the line shape is built:
+ * NShapes::createLine(..., referenceCompat).) */
 enum {
   /* ExpQMBarrierSM.cu:111 overwrites the surface normal with the electron
    * direction: every barrier hit is at normal incidence. */

@@ -6,9 +6,10 @@ The log is a 0.5 keV PMMA line scan with 100 primaries per pixel from an
 unrecorded revision of the reference; far from the line (y > 700 nm) the
 surface is flat PMMA.  Its BSE yield is compared within 4 standard errors,
 its SE yield -- counted as E <= 40 eV through the histogram defect -- within
-2 % (the oracle sits 1.3 % above the log, independent of the random-number
-generator model; see the design notes).  Smoke-level, as the log's revision
-is unknown; the 3-sigma gates are GPU against the oracle."""
+2 % (the oracle sits 1.3 % above the log whichever random-number generator
+model it uses: Philox 2.2135, 15-bit uniforms 2.2200, the MSVC rand() LCG
+2.2176, log 2.1888).  Smoke-level, as the log's revision is unknown; the
+3-sigma gates are GPU against the oracle."""
 import json
 import os
 
