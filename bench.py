@@ -42,6 +42,21 @@ def build_scene():
     return sc, centers
 
 
+def copy_calibration(nbytes):
+    """One device-to-device copy of nbytes through the HIP runtime the
+    engine uses, so that a counter session sees a kernel of known traffic."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def cpu_baseline(sc, centers, target_s=10.0):
     """Time the oracle on a bounded sample of the scan (all 128 pixels, a few
     primaries each), sized to about target_s seconds of wall time."""
@@ -83,12 +98,7 @@ def main():
     import cudamonsel_amd as M
 
     if args.calib_copy_mb > 0:
-        import torch
-        a = torch.empty(args.calib_copy_mb << 20, dtype=torch.uint8, device="cuda:0")
-        b = torch.empty_like(a)
-        b.copy_(a)
-        torch.cuda.synchronize()
-        del a, b
+        copy_calibration(args.calib_copy_mb << 20)
 
     sc, centers = build_scene()
     monte = sc.monte

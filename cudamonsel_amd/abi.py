@@ -40,6 +40,14 @@ class JobStats(ctypes.Structure):
     ]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [
+        ("block_size", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32), ("max_generations", ctypes.c_int32),
+        ("queue_capacity", ctypes.c_int32), ("max_jobs_in_flight", ctypes.c_int32),
+        ("primaries_per_launch", ctypes.c_int32), ("steps_per_launch", ctypes.c_int32),
+    ]
+
+
 SYMBOLS = [
     "monsel_last_error", "monsel_abi_version", "monsel_create", "monsel_destroy", "monsel_device_info",
     "monsel_scene_footprint", "monsel_scene_upload", "monsel_scene_release", "monsel_job_footprint",
@@ -146,3 +154,66 @@ def probe_for(scene_obj, device=0):
     dev = Device.get(device)
     sid = dev.upload(cs)
     return Probe(dev.ctx_address(), sid), cs
+
+
+class Engine:
+    """A device context of its own (with launch tuning) driven through the
+    C-ABI: scene upload, asynchronous job submission, results as arrays."""
+
+    def __init__(self, device=0, **tuning):
+        self.L = gpu_lib()
+        self.ctx = ctypes.c_void_p()
+        t = Tuning(**tuning)
+        _check(self.L.monsel_create(int(device), ctypes.byref(t), ctypes.byref(self.ctx)), "monsel_create")
+        self._keep = {}
+
+    def close(self):
+        if self.ctx:
+            self.L.monsel_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def upload(self, compiled):
+        sid = ctypes.c_uint64()
+        _check(self.L.monsel_scene_upload(self.ctx, ctypes.c_void_p(compiled.desc_address()), ctypes.byref(sid)),
+               "monsel_scene_upload")
+        return sid.value
+
+    def submit(self, scene_id, centers_m, per_pixel, beam_eV, sigma_m, seed, nbins=50, se_cutoff_eV=50.0,
+               max_steps=0, primary_offset=0):
+        c = np.ascontiguousarray(centers_m, dtype=np.float64).reshape(-1, 3)
+        jd = JobDesc(beam_eV * 1.60217653e-19, sigma_m, c.shape[0], int(per_pixel), c.ctypes.data_as(c_double_p),
+                     int(seed), int(nbins), int(max_steps), float(se_cutoff_eV), int(primary_offset))
+        jid = ctypes.c_uint64()
+        _check(self.L.monsel_job_submit(self.ctx, ctypes.c_uint64(scene_id), ctypes.byref(jd), ctypes.byref(jid)),
+               "monsel_job_submit")
+        self._keep[jid.value] = (c.shape[0], int(nbins))
+        return jid.value
+
+    def result(self, job_id, timeout_s=None):
+        """Waits for the job, returns its tallies and statistics, releases it."""
+        us = -1 if timeout_s is None else int(timeout_s * 1e6)
+        _check(self.L.monsel_job_wait(self.ctx, ctypes.c_uint64(job_id), ctypes.c_int64(us)), "monsel_job_wait")
+        n, nb = self._keep.pop(job_id)
+        r = {k: np.zeros(n, np.uint64) for k in ("back", "fwd", "se")}
+        r.update({k: np.zeros(n) for k in ("se_sq", "bse_sq", "back_energy")})
+        r["back_energy_hist"] = np.zeros(nb + 2, np.uint64)
+        r["fwd_energy_hist"] = np.zeros(nb + 2, np.uint64)
+        r["elevation_hist"] = np.zeros(182, np.uint64)
+        r["azimuth_hist"] = np.zeros(362, np.uint64)
+        t = Tallies()
+        for k in ("back", "fwd", "se", "back_energy_hist", "fwd_energy_hist", "elevation_hist", "azimuth_hist"):
+            setattr(t, k, r[k].ctypes.data_as(c_u64_p))
+        for k in ("se_sq", "bse_sq", "back_energy"):
+            setattr(t, k, r[k].ctypes.data_as(c_double_p))
+        _check(self.L.monsel_job_read(self.ctx, ctypes.c_uint64(job_id), ctypes.byref(t)), "monsel_job_read")
+        st = JobStats()
+        _check(self.L.monsel_job_stats_get(self.ctx, ctypes.c_uint64(job_id), ctypes.byref(st)), "monsel_job_stats_get")
+        r["stats"] = {f: getattr(st, f) for f, _ in JobStats._fields_}
+        self.L.monsel_job_release(self.ctx, ctypes.c_uint64(job_id))
+        return r

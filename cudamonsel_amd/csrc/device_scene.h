@@ -93,9 +93,10 @@ struct DMech {
   int table[4];
   float p[24];
   DCSD csd; /* FittedInelSM's slowing-down algorithm */
+  double q[24]; /* tabulated mechanism: the TP_* slots in SI units (J) for the FP64 scatter */
 };
 
-/* Tabulated-mechanism parameter slots in DMech::p. */
+/* Tabulated-mechanism parameter slots in DMech::p (eV) and DMech::q (J). */
 enum {
   TP_OFFSET = 0, TP_MINEGEN = 1, TP_RATEMULT = 2, TP_E0DISP = 3, TP_CB = 4, TP_BANDGAP = 5,
   TP_GAP = 6, TP_OFFSETFERMI = 7, TP_BEREF = 8, TP_EIDOM0 = 9, TP_EIDOM1 = 10, TP_IIMFPDOM0 = 11,
@@ -140,6 +141,7 @@ struct DevScene {
   const float* mott_x1;    /* [n_mott][61*201] */
   const DNUTable* nu;
   const float* nu_data;
+  const double* nu_data64; /* the same values and abscissae in FP64 */
 };
 
 }  // namespace monsel_dev

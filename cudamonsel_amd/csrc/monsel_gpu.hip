@@ -71,41 +71,54 @@ struct MonselError {
 /* ===================================================================== */
 
 constexpr int kQF = 7; /* float fields per queued electron: p, d, E */
-constexpr int kQU = 4; /* u32 fields: region, primary, lineage lo/hi */
+constexpr int kQU = 7; /* u32 fields: region, primary, lineage lo/hi, job slot, steps, children */
 constexpr int kEF = 7; /* float fields per escape record: tally origin, d, E */
-constexpr int kEU = 2; /* u32 fields: primary, through-the-chamber-wall flag */
+constexpr int kEU = 2; /* u32 fields: primary, job slot << 1 | through-the-chamber-wall flag */
 constexpr int kStatSteps = 0, kStatElectrons = 1, kStatEscapes = 2, kStatWatchdog = 3, kStatOverflow = 4;
+constexpr int kMaxJobs = 1024; /* job-table slots of one context */
+/* Back-escape energies are summed as integers in units of 2^-20 eV, so that
+ * the sums are exact and independent of the order of the atomics. */
+constexpr double kEnergyScale = 1048576.0;
 
-struct JobDev {
-  uint32_t k0, k1;
-  float E0;         /* eV */
-  float width;      /* nm */
-  const double* centers; /* m */
+/* One job's device record (a slot of the context's job table).  Every
+ * electron carries its job's slot; jobs of one batch share the transport
+ * launches and differ only in these records. */
+struct JobRec {
+  uint64_t lin0;          /* lineage of the job's primaries (from its seed) */
+  float E0;               /* eV */
+  float width;            /* nm */
+  const double* centers;  /* m, [n_pixels][3] */
   int n_pixels;
   int ppp;
-  uint32_t prim_base;   /* RNG id of the chunk's first primary */
-  uint32_t chunk_start; /* job-relative index of the chunk's first primary */
-  uint32_t chunk_n;
-  int nbins;
-  double E0eV;
-  int maxSEbin;
+  uint32_t prim_base;     /* random-stream index of the job's primary 0 */
+  int root;               /* the job's chamber region in the batch scene */
   int max_steps;
-  uint32_t* prim_counts; /* [chunk_n]: low 16 bits SE, high 16 bits BSE */
-  unsigned long long* fwd_px;
-  double* back_energy_px;
-  unsigned long long* hist_back;
-  unsigned long long* hist_fwd;
-  unsigned long long* hist_elev;
-  unsigned long long* hist_azim;
-  unsigned long long* stats;
+  int nbins;
+  int maxSEbin;
+  int killed;             /* set by the host: drop the job's electrons */
+  double E0eV;
+  double chamber_c[3];    /* nm */
+  double chamber_r;       /* nm */
+  uint32_t* prim_counts;  /* [primaries]: back-escapes, low 16 bits SE, high 16 bits BSE */
+  uint32_t* prim_fwd;     /* [primaries]: forward escapes */
+  unsigned long long* prim_energy; /* [primaries]: summed back-escape energy, units of kEnergyQuantum eV */
+  unsigned long long* hist; /* back energy [nbins+2], fwd energy [nbins+2], elevation [182], azimuth [362] */
+  unsigned long long* stats; /* [8] */
 };
 
+/* One transport launch ("generation"): new primaries described by
+ * segments, then the queued electrons of the previous launch. */
 struct GenDev {
-  int mode; /* 0: create primaries; 1: read queue */
+  const JobRec* jobs;
+  const uint32_t* seg_end;   /* [n_seg] running item count at the end of each segment */
+  const uint32_t* seg_job;   /* [n_seg] job slot */
+  const uint32_t* seg_first; /* [n_seg] first job-relative primary index */
+  int n_seg;
+  uint32_t n_prim;           /* new-primary items (= seg_end[n_seg-1]) */
   const float* in_f;
   const uint32_t* in_u;
   uint32_t in_cap;
-  uint32_t in_n; /* electrons in the input (mode 1) */
+  uint32_t in_n;
   unsigned int* claim;
   float* out_f;
   uint32_t* out_u;
@@ -116,6 +129,7 @@ struct GenDev {
   uint32_t* esc_u;
   uint32_t esc_cap;
   unsigned int* esc_count;
+  int steps_per_launch;      /* regroup interval: steps before an electron returns to the queue (0 = none) */
 };
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
@@ -138,19 +152,25 @@ __device__ __forceinline__ int hist_bin(double val, double min, double max, int 
   return l - 1;
 }
 
+/* Offsets of the four histograms of a job (Histogram.cu:25 slot layout:
+ * 0 underrange, 1..n, n+1 overrange). */
+__host__ __device__ __forceinline__ int hist_len(int nbins) { return 2 * (nbins + 2) + 182 + 362; }
+
 /* BackscatterEvent -> BackscatterStats::actionPerformed (BackscatterStats.cu:147-170)
- * at the chamber wall point reached from (px,py,pz) along (dx,dy,dz). */
-__device__ __forceinline__ void tally_escape(const DevScene& S, const JobDev& J, const Elec& e, float px, float py, float pz,
-                             float dx, float dy, float dz, bool fromRoot) {
+ * at the chamber wall point reached from (px,py,pz) along (dx,dy,dz).
+ * Histogram counts go to `lh` (a block-private LDS copy) when given. */
+__device__ __forceinline__ void tally_escape(const DevScene& S, const JobRec& J, uint32_t primary, float E, float px,
+                                             float py, float pz, float dx, float dy, float dz, bool fromRoot,
+                                             unsigned int* lh) {
   double x = px, y = py, z = pz;
   if (fromRoot) {
-    double cx = x - (double)S.chamber_c[0], cy = y - (double)S.chamber_c[1], cz = z - (double)S.chamber_c[2];
+    double cx = x - J.chamber_c[0], cy = y - J.chamber_c[1], cz = z - J.chamber_c[2];
     double ddx = dx, ddy = dy, ddz = dz;
     double dn = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
     ddx /= dn;
     ddy /= dn;
     ddz /= dn;
-    double R = (double)S.chamber_r;
+    double R = J.chamber_r;
     double b = cx * ddx + cy * ddy + cz * ddz;
     double c = cx * cx + cy * cy + cz * cz - R * R;
     double disc = b * b - c;
@@ -164,20 +184,29 @@ __device__ __forceinline__ void tally_escape(const DevScene& S, const JobDev& J,
   const double elevation = (PI / 2) - atan2(z, sqrt(x * x + y * y));
   double azimuth = atan2(y, x);
   if (azimuth < 0.0) azimuth = (2.0 * PI) + azimuth;
-  atomicAdd(&J.hist_elev[hist_bin(elevation, 0.0, PI, 180, compat) + 1], 1ull);
-  atomicAdd(&J.hist_azim[hist_bin(azimuth, 0.0, 2.0 * PI, 360, compat) + 1], 1ull);
-  const double kEeV = (double)e.E;
-  int bin = hist_bin(kEeV, 0.0, J.E0eV, J.nbins, compat);
-  uint32_t rel = e.primary - J.prim_base;
-  uint32_t pix = (J.chunk_start + rel) / (uint32_t)J.ppp;
-  if (elevation < (PI / 2.0)) {
-    atomicAdd(&J.hist_fwd[bin + 1], 1ull);
-    atomicAdd(&J.fwd_px[pix], 1ull);
+  const int nb2 = J.nbins + 2;
+  const double kEeV = (double)E;
+  const int bin = hist_bin(kEeV, 0.0, J.E0eV, J.nbins, compat);
+  const bool fwd = elevation < (PI / 2.0);
+  const int o_elev = 2 * nb2 + hist_bin(elevation, 0.0, PI, 180, compat) + 1;
+  const int o_azim = 2 * nb2 + 182 + hist_bin(azimuth, 0.0, 2.0 * PI, 360, compat) + 1;
+  const int o_e = (fwd ? nb2 : 0) + bin + 1;
+  if (lh) {
+    atomicAdd(&lh[o_elev], 1u);
+    atomicAdd(&lh[o_azim], 1u);
+    atomicAdd(&lh[o_e], 1u);
   } else {
-    atomicAdd(&J.hist_back[bin + 1], 1ull);
-    atomicAdd(&J.back_energy_px[pix], kEeV);
-    bool se = bin >= 0 && bin < J.maxSEbin;
+    atomicAdd(&J.hist[o_elev], 1ull);
+    atomicAdd(&J.hist[o_azim], 1ull);
+    atomicAdd(&J.hist[o_e], 1ull);
+  }
+  const uint32_t rel = primary - J.prim_base;
+  if (fwd) {
+    atomicAdd(&J.prim_fwd[rel], 1u);
+  } else {
+    const bool se = bin >= 0 && bin < J.maxSEbin;
     atomicAdd(&J.prim_counts[rel], se ? 1u : 0x10000u);
+    atomicAdd(&J.prim_energy[rel], (unsigned long long)__double2ll_rn((double)E * kEnergyScale));
   }
 }
 
@@ -258,17 +287,17 @@ __device__ __forceinline__ int barrier_core(const DevScene& S, const DMSM& msm, 
 }
 
 /* One MonteCarloSS::takeStep (MonteCarloSS.cu:107-163). */
-__device__ __forceinline__ int take_step(const DevScene& S, const JobDev& J, Elec& e, Spawn& sp) {
+__device__ __forceinline__ int take_step(const DevScene& S, const JobRec* jobs, Elec& e, Spawn& sp, int maxSteps) {
   sp.has = false;
   int region = e.region;
   if (region < 0 || !shape_contains(S, S.regions[region].shape, e.px, e.py, e.pz)) {
-    region = containing_sub_region(S, 0, e.px, e.py, e.pz);
+    region = containing_sub_region(S, jobs[e.job].root, e.px, e.py, e.pz);
     e.region = region;
     if (region < 0) return STEP_DEAD;
   }
   const DMSM& msm = S.msms[S.regions[region].msm];
   float u[kStepUniforms];
-  step_uniforms(e, J.k0, J.k1, u);
+  step_uniforms(e, kPhiloxKey0, kPhiloxKey1, u);
   /* MONSEL_MaterialScatterModel::randomMeanPathLength (:64-91); each
    * mechanism's rate is kept in a per-lane LDS slot for the scatter choice */
   __shared__ float l_rate[kMaxMech][kLaneBlock];
@@ -300,7 +329,7 @@ __device__ __forceinline__ int take_step(const DevScene& S, const JobDev& J, Ele
   float prevE = e.E;
   if (!msm.is_null) e.E += csd_compute(msm.csd, len, e.E);
   e.steps++;
-  if ((int)e.steps > J.max_steps || !(e.E == e.E)) return STEP_WATCHDOG;
+  if ((int)e.steps > maxSteps || !(e.E == e.E)) return STEP_WATCHDOG;
   if (e.E < msm.minE) return STEP_DEAD;
   if (se.next == region) {
     /* MONSEL_MaterialScatterModel::scatter (:93-134), rates at prevE */
@@ -372,17 +401,18 @@ __device__ __forceinline__ int take_step(const DevScene& S, const JobDev& J, Ele
   return STEP_ESCAPED;
 }
 
-__device__ __forceinline__ void make_primary(const DevScene& S, const JobDev& J, uint32_t idx, Elec& e) {
-  uint32_t job_idx = J.chunk_start + idx;
-  uint32_t pix = job_idx / (uint32_t)J.ppp;
+/* GaussianBeam::createElectron (GaussianBeam.cu:52-65) for primary `idx` of
+ * job slot `job`. */
+__device__ __forceinline__ void make_primary(const DevScene& S, const JobRec& J, uint32_t job, uint32_t idx, Elec& e) {
+  const uint32_t pix = idx / (uint32_t)J.ppp;
   e.primary = J.prim_base + idx;
-  e.lin_lo = 0;
-  e.lin_hi = 0;
+  e.lin_lo = (uint32_t)J.lin0;
+  e.lin_hi = (uint32_t)(J.lin0 >> 32);
+  e.job = job;
   e.nchild = 0;
   e.steps = 0;
-  /* GaussianBeam::createElectron (GaussianBeam.cu:52-65) */
   uint32_t a0, a1, a2, a3;
-  philox4x32_10(kPrimaryBlock, e.lin_lo, e.lin_hi, e.primary, J.k0, J.k1, a0, a1, a2, a3);
+  philox4x32_10(kPrimaryBlock, e.lin_lo, e.lin_hi, e.primary, kPhiloxKey0, kPhiloxKey1, a0, a1, a2, a3);
   const float u1 = u01(a0), u2 = u01(a1);
   float r = __builtin_sqrtf(-2.f * __logf(u1)) * J.width;
   float s, c;
@@ -395,7 +425,53 @@ __device__ __forceinline__ void make_primary(const DevScene& S, const JobDev& J,
   e.dy = 0.f;
   e.dz = 1.f;
   e.E = J.E0;
-  e.region = containing_sub_region(S, 0, e.px, e.py, e.pz);
+  e.region = containing_sub_region(S, J.root, e.px, e.py, e.pz);
+}
+
+/* Appends electrons to the next launch's queue: one atomic per wave. */
+__device__ __forceinline__ void queue_append(const GenDev& G, bool has, float px, float py, float pz, float dx, float dy,
+                                             float dz, float E, int region, uint32_t primary, uint32_t lin_lo,
+                                             uint32_t lin_hi, uint32_t job, uint32_t steps, uint32_t nchild,
+                                             unsigned long long lanemask_lt, uint32_t& nover) {
+  const unsigned long long m = wave_ballot(has);
+  if (!m) return;
+  const int leader = __ffsll((long long)m) - 1;
+  uint32_t base = 0;
+  if (lane_id() == leader) base = atomicAdd(G.out_count, (unsigned int)__popcll(m));
+  base = __shfl(base, leader);
+  if (!has) return;
+  const uint32_t slot = base + (uint32_t)__popcll(m & lanemask_lt);
+  if (slot >= G.out_cap) {
+    nover++;
+    return;
+  }
+  const uint32_t cap = G.out_cap;
+  G.out_f[0 * cap + slot] = px;
+  G.out_f[1 * cap + slot] = py;
+  G.out_f[2 * cap + slot] = pz;
+  G.out_f[3 * cap + slot] = dx;
+  G.out_f[4 * cap + slot] = dy;
+  G.out_f[5 * cap + slot] = dz;
+  G.out_f[6 * cap + slot] = E;
+  G.out_u[0 * cap + slot] = (uint32_t)region;
+  G.out_u[1 * cap + slot] = primary;
+  G.out_u[2 * cap + slot] = lin_lo;
+  G.out_u[3 * cap + slot] = lin_hi;
+  G.out_u[4 * cap + slot] = job;
+  G.out_u[5 * cap + slot] = steps;
+  G.out_u[6 * cap + slot] = nchild;
+}
+
+/* adds one lane's counters to its job's statistics (per-lane atomics) */
+__device__ __forceinline__ void flush_lane_stats(const GenDev& G, uint32_t job, uint32_t& nsteps, uint32_t& nelec,
+                                                 uint32_t& nesc, uint32_t& nwd, uint32_t& nover) {
+  unsigned long long* st = G.jobs[job].stats;
+  if (nsteps) atomicAdd(&st[kStatSteps], (unsigned long long)nsteps);
+  if (nelec) atomicAdd(&st[kStatElectrons], (unsigned long long)nelec);
+  if (nesc) atomicAdd(&st[kStatEscapes], (unsigned long long)nesc);
+  if (nwd) atomicAdd(&st[kStatWatchdog], (unsigned long long)nwd);
+  if (nover) atomicAdd(&st[kStatOverflow], (unsigned long long)nover);
+  nsteps = nelec = nesc = nwd = nover = 0;
 }
 
 /* waves per SIMD the register allocation of k_transport targets */
@@ -403,11 +479,14 @@ __device__ __forceinline__ void make_primary(const DevScene& S, const JobDev& J,
 #define MONSEL_TRANSPORT_WAVES_PER_EU 4
 #endif
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MONSEL_TRANSPORT_WAVES_PER_EU)))
-k_transport(DevScene S, JobDev J, GenDev G) {
+k_transport(DevScene S, GenDev G) {
   Elec e;
   bool alive = false, done = false;
+  int maxSteps = 0;
+  uint32_t steps0 = 0;                 /* step count when the lane took the electron */
+  uint32_t cur_job = 0xFFFFFFFFu;      /* job the lane's counters belong to */
   uint32_t nsteps = 0, nelec = 0, nesc = 0, nwd = 0, nover = 0;
-  const uint32_t in_count = G.mode == 0 ? J.chunk_n : G.in_n;
+  const uint32_t n_items = G.n_prim + G.in_n;
   const unsigned long long lanemask_lt = (1ull << lane_id()) - 1ull;
   for (;;) {
     /* claim work for idle lanes: one atomic per wave */
@@ -419,28 +498,50 @@ k_transport(DevScene S, JobDev J, GenDev G) {
       if (lane_id() == leader) base = atomicAdd(G.claim, (unsigned int)__popcll(m));
       base = __shfl(base, leader);
       if (need) {
-        uint32_t idx = base + (uint32_t)__popcll(m & lanemask_lt);
-        if (idx < in_count) {
-          if (G.mode == 0) {
-            make_primary(S, J, idx, e);
+        const uint32_t idx = base + (uint32_t)__popcll(m & lanemask_lt);
+        if (idx < n_items) {
+          if (idx < G.n_prim) {
+            /* segment holding item idx (binary search over the running ends) */
+            int lo = 0, hi = G.n_seg - 1;
+            while (lo < hi) {
+              const int mid = (lo + hi) >> 1;
+              if (G.seg_end[mid] > idx) hi = mid;
+              else lo = mid + 1;
+            }
+            const uint32_t start = lo > 0 ? G.seg_end[lo - 1] : 0u;
+            const uint32_t job = G.seg_job[lo];
+            make_primary(S, G.jobs[job], job, G.seg_first[lo] + (idx - start), e);
           } else {
+            const uint32_t q = idx - G.n_prim;
             const uint32_t cap = G.in_cap;
-            e.px = G.in_f[0 * cap + idx];
-            e.py = G.in_f[1 * cap + idx];
-            e.pz = G.in_f[2 * cap + idx];
-            e.dx = G.in_f[3 * cap + idx];
-            e.dy = G.in_f[4 * cap + idx];
-            e.dz = G.in_f[5 * cap + idx];
-            e.E = G.in_f[6 * cap + idx];
-            e.region = (int)G.in_u[0 * cap + idx];
-            e.primary = G.in_u[1 * cap + idx];
-            e.lin_lo = G.in_u[2 * cap + idx];
-            e.lin_hi = G.in_u[3 * cap + idx];
-            e.nchild = 0;
-            e.steps = 0;
+            e.px = G.in_f[0 * cap + q];
+            e.py = G.in_f[1 * cap + q];
+            e.pz = G.in_f[2 * cap + q];
+            e.dx = G.in_f[3 * cap + q];
+            e.dy = G.in_f[4 * cap + q];
+            e.dz = G.in_f[5 * cap + q];
+            e.E = G.in_f[6 * cap + q];
+            e.region = (int)G.in_u[0 * cap + q];
+            e.primary = G.in_u[1 * cap + q];
+            e.lin_lo = G.in_u[2 * cap + q];
+            e.lin_hi = G.in_u[3 * cap + q];
+            e.job = G.in_u[4 * cap + q];
+            e.steps = G.in_u[5 * cap + q];
+            e.nchild = G.in_u[6 * cap + q];
           }
-          alive = true;
-          nelec++;
+          if (e.job != cur_job) {
+            if (cur_job != 0xFFFFFFFFu) flush_lane_stats(G, cur_job, nsteps, nelec, nesc, nwd, nover);
+            cur_job = e.job;
+          }
+          const JobRec& J = G.jobs[e.job];
+          if (J.killed) {
+            nover++;
+          } else {
+            maxSteps = J.max_steps;
+            steps0 = e.steps;
+            alive = true;
+            if (e.steps == 0) nelec++;
+          }
         } else {
           done = true;
         }
@@ -451,38 +552,24 @@ k_transport(DevScene S, JobDev J, GenDev G) {
     sp.has = false;
     int res = STEP_CONTINUE;
     if (alive) {
-      res = take_step(S, J, e, sp);
+      res = take_step(S, G.jobs, e, sp, maxSteps);
       nsteps++;
     }
-    /* append secondaries: one atomic per wave */
-    unsigned long long sm = wave_ballot(sp.has);
-    if (sm) {
-      int leader = __ffsll((long long)sm) - 1;
-      uint32_t base = 0;
-      if (lane_id() == leader) base = atomicAdd(G.out_count, (unsigned int)__popcll(sm));
-      base = __shfl(base, leader);
-      if (sp.has) {
-        uint32_t slot = base + (uint32_t)__popcll(sm & lanemask_lt);
-        uint64_t lin = child_lineage(((uint64_t)e.lin_hi << 32) | e.lin_lo, e.nchild++);
-        if (slot < G.out_cap) {
-          const uint32_t cap = G.out_cap;
-          G.out_f[0 * cap + slot] = e.px;
-          G.out_f[1 * cap + slot] = e.py;
-          G.out_f[2 * cap + slot] = e.pz;
-          G.out_f[3 * cap + slot] = sp.dx;
-          G.out_f[4 * cap + slot] = sp.dy;
-          G.out_f[5 * cap + slot] = sp.dz;
-          G.out_f[6 * cap + slot] = sp.E;
-          G.out_u[0 * cap + slot] = (uint32_t)e.region;
-          G.out_u[1 * cap + slot] = e.primary;
-          G.out_u[2 * cap + slot] = (uint32_t)lin;
-          G.out_u[3 * cap + slot] = (uint32_t)(lin >> 32);
-        } else {
-          nover++;
-        }
-      }
+    /* secondaries to the next launch */
+    uint32_t clo = 0, chi = 0;
+    if (sp.has) {
+      const uint64_t lin = child_lineage(((uint64_t)e.lin_hi << 32) | e.lin_lo, e.nchild++);
+      clo = (uint32_t)lin;
+      chi = (uint32_t)(lin >> 32);
     }
-    /* append escape records: one atomic per wave */
+    queue_append(G, sp.has, e.px, e.py, e.pz, sp.dx, sp.dy, sp.dz, sp.E, e.region, e.primary, clo, chi, e.job, 0u, 0u,
+                 lanemask_lt, nover);
+    /* regroup: an electron past its step budget for this launch continues in the next one */
+    const bool requeue = alive && res == STEP_CONTINUE && G.steps_per_launch > 0 &&
+                         (int)(e.steps - steps0) >= G.steps_per_launch;
+    queue_append(G, requeue, e.px, e.py, e.pz, e.dx, e.dy, e.dz, e.E, e.region, e.primary, e.lin_lo, e.lin_hi, e.job,
+                 e.steps, e.nchild, lanemask_lt, nover);
+    /* escape records: one atomic per wave */
     const bool esc = alive && res == STEP_ESCAPED;
     unsigned long long em = wave_ballot(esc);
     if (em) {
@@ -502,82 +589,131 @@ k_transport(DevScene S, JobDev J, GenDev G) {
           G.esc_f[5 * cap + slot] = e.dz;
           G.esc_f[6 * cap + slot] = e.E;
           G.esc_u[0 * cap + slot] = e.primary;
-          G.esc_u[1 * cap + slot] = (uint32_t)e.region;
+          G.esc_u[1 * cap + slot] = (e.job << 1) | (uint32_t)e.region;
         } else {
           nover++;
         }
       }
     }
-    if (alive && res != STEP_CONTINUE) {
+    if (alive && (res != STEP_CONTINUE || requeue)) {
       alive = false;
       if (res == STEP_ESCAPED) nesc++;
       if (res == STEP_WATCHDOG) nwd++;
     }
   }
-  /* flush counters: lane sums reduced across the wave, one atomic each */
-  unsigned long long wsteps = nsteps, welec = nelec, wesc = nesc, wwd = nwd, wover = nover;
-  for (int off = 32; off > 0; off >>= 1) {
-    wsteps += __shfl_down(wsteps, off);
-    welec += __shfl_down(welec, off);
-    wesc += __shfl_down(wesc, off);
-    wwd += __shfl_down(wwd, off);
-    wover += __shfl_down(wover, off);
-  }
-  if (lane_id() == 0) {
-    atomicAdd(&J.stats[kStatSteps], wsteps);
-    atomicAdd(&J.stats[kStatElectrons], welec);
-    if (wesc) atomicAdd(&J.stats[kStatEscapes], wesc);
-    if (wwd) atomicAdd(&J.stats[kStatWatchdog], wwd);
-    if (wover) atomicAdd(&J.stats[kStatOverflow], wover);
+  /* flush counters: reduced across the wave when every lane's counters
+   * belong to one job, per lane otherwise */
+  const unsigned long long valid = wave_ballot(cur_job != 0xFFFFFFFFu);
+  if (!valid) return;
+  const uint32_t j0 = __shfl(cur_job, __ffsll((long long)valid) - 1);
+  if (wave_ballot(cur_job != 0xFFFFFFFFu && cur_job != j0) == 0ull) {
+    unsigned long long wsteps = nsteps, welec = nelec, wesc = nesc, wwd = nwd, wover = nover;
+    for (int off = 32; off > 0; off >>= 1) {
+      wsteps += __shfl_down(wsteps, off);
+      welec += __shfl_down(welec, off);
+      wesc += __shfl_down(wesc, off);
+      wwd += __shfl_down(wwd, off);
+      wover += __shfl_down(wover, off);
+    }
+    if (lane_id() == 0) {
+      unsigned long long* st = G.jobs[j0].stats;
+      atomicAdd(&st[kStatSteps], wsteps);
+      atomicAdd(&st[kStatElectrons], welec);
+      if (wesc) atomicAdd(&st[kStatEscapes], wesc);
+      if (wwd) atomicAdd(&st[kStatWatchdog], wwd);
+      if (wover) atomicAdd(&st[kStatOverflow], wover);
+    }
+  } else if (cur_job != 0xFFFFFFFFu) {
+    flush_lane_stats(G, cur_job, nsteps, nelec, nesc, nwd, nover);
   }
 }
 
 /* Tallies of the escape records of one transport launch (grid-stride over
- * the record count the launch left in *count). */
-__global__ void __launch_bounds__(256) k_tally_escapes(DevScene S, JobDev J, const float* ef, const uint32_t* eu,
-                                                       uint32_t cap, const unsigned int* count) {
+ * the record count the launch left in *count).  The histograms of job slot
+ * `lds_job` accumulate in a block-private LDS copy (dynamic shared memory of
+ * hist_len(nbins) words) that is added to the job's histograms at the end;
+ * other jobs' records use global atomics. */
+__global__ void __launch_bounds__(256) k_tally_escapes(DevScene S, const JobRec* jobs, const float* ef,
+                                                       const uint32_t* eu, uint32_t cap, const unsigned int* count,
+                                                       int lds_job) {
+  extern __shared__ unsigned int lh[];
+  const int L = lds_job >= 0 ? hist_len(jobs[lds_job].nbins) : 0;
+  for (int i = threadIdx.x; i < L; i += blockDim.x) lh[i] = 0u;
+  __syncthreads();
   const uint32_t n = min(*count, cap);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    Elec e;
-    e.E = ef[6 * cap + i];
-    e.primary = eu[i];
-    tally_escape(S, J, e, ef[i], ef[cap + i], ef[2 * cap + i], ef[3 * cap + i], ef[4 * cap + i], ef[5 * cap + i],
-                 eu[cap + i] != 0u);
+    const uint32_t w = eu[cap + i];
+    const int job = (int)(w >> 1);
+    tally_escape(S, jobs[job], eu[i], ef[6 * cap + i], ef[i], ef[cap + i], ef[2 * cap + i], ef[3 * cap + i],
+                 ef[4 * cap + i], ef[5 * cap + i], (w & 1u) != 0u, job == lds_job ? lh : nullptr);
+  }
+  __syncthreads();
+  if (L > 0) {
+    unsigned long long* h = jobs[lds_job].hist;
+    for (int i = threadIdx.x; i < L; i += blockDim.x)
+      if (lh[i]) atomicAdd(&h[i], (unsigned long long)lh[i]);
   }
 }
 
-/* Per-primary SE/BSE counts -> per-pixel sums of n and n^2 (exact u64).
- * Primaries of one pixel are consecutive, so a wave whose 64 primaries share
- * a pixel reduces in registers and issues one atomic per sum; a wave that
- * straddles pixels falls back to per-lane atomics. */
-__global__ void k_reduce_primaries(JobDev J, unsigned long long* back_px, unsigned long long* se_px,
-                                   unsigned long long* se_sq, unsigned long long* bse_sq) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = i < J.chunk_n;
-  uint32_t c = valid ? J.prim_counts[i] : 0u;
-  uint32_t iv = valid ? i : J.chunk_n - 1u;
+/* Electrons per job slot in a launch's output queue (block-private LDS
+ * counts of up to kMaxJobs slots, then one atomic per slot and block). */
+__global__ void __launch_bounds__(256) k_count_jobs(const uint32_t* job_field, const unsigned int* count, uint32_t cap,
+                                                    unsigned int* per_job, int n_slots) {
+  __shared__ unsigned int lc[kMaxJobs];
+  for (int i = threadIdx.x; i < n_slots; i += blockDim.x) lc[i] = 0u;
+  __syncthreads();
+  const uint32_t n = min(*count, cap);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t j = job_field[i];
+    if (j < (uint32_t)n_slots) atomicAdd(&lc[j], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n_slots; i += blockDim.x)
+    if (lc[i]) atomicAdd(&per_job[i], lc[i]);
+}
+
+/* Per-primary tallies of one job -> per-pixel sums (exact u64 counts,
+ * second moments and fixed-point energy).  px holds back, fwd, se, se^2,
+ * bse^2, energy ([6][n_pixels]).  Primaries of one pixel are consecutive, so a wave whose
+ * 64 primaries share a pixel reduces in registers and issues one atomic per
+ * sum; a wave that straddles pixels falls back to per-lane atomics. */
+__global__ void k_reduce_primaries(const JobRec* jobs, int slot, uint32_t n, unsigned long long* px) {
+  const JobRec& J = jobs[slot];
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < n;
+  const uint32_t iv = valid ? i : n - 1u;
+  const uint32_t c = valid ? J.prim_counts[i] : 0u;
   unsigned long long nse = c & 0xFFFFu, nbse = c >> 16;
-  uint32_t pix = (J.chunk_start + iv) / (uint32_t)J.ppp;
+  unsigned long long vf = valid ? J.prim_fwd[i] : 0u;
+  unsigned long long ve = valid ? J.prim_energy[i] : 0ull;
+  const uint32_t pix = iv / (uint32_t)J.ppp;
   unsigned long long vb = nse + nbse, vs = nse, vss = nse * nse, vbb = nbse * nbse;
+  const uint32_t np = (uint32_t)J.n_pixels;
   const uint32_t pix0 = __shfl(pix, 0);
   if (wave_ballot(pix != pix0) == 0ull) {
     for (int off = 32; off > 0; off >>= 1) {
       vb += __shfl_xor(vb, off);
+      vf += __shfl_xor(vf, off);
       vs += __shfl_xor(vs, off);
       vss += __shfl_xor(vss, off);
       vbb += __shfl_xor(vbb, off);
+      ve += __shfl_xor(ve, off);
     }
     if (lane_id() == 0) {
-      if (vb) atomicAdd(&back_px[pix0], vb);
-      if (vs) atomicAdd(&se_px[pix0], vs);
-      if (vss) atomicAdd(&se_sq[pix0], vss);
-      if (vbb) atomicAdd(&bse_sq[pix0], vbb);
+      if (vb) atomicAdd(&px[pix0], vb);
+      if (vf) atomicAdd(&px[np + pix0], vf);
+      if (vs) atomicAdd(&px[2 * np + pix0], vs);
+      if (vss) atomicAdd(&px[3 * np + pix0], vss);
+      if (vbb) atomicAdd(&px[4 * np + pix0], vbb);
+      if (ve) atomicAdd(&px[5 * np + pix0], ve);
     }
-  } else if (vb) {
-    atomicAdd(&back_px[pix], vb);
-    if (vs) atomicAdd(&se_px[pix], vs);
-    if (vss) atomicAdd(&se_sq[pix], vss);
-    if (vbb) atomicAdd(&bse_sq[pix], vbb);
+  } else {
+    if (vb) atomicAdd(&px[pix], vb);
+    if (vf) atomicAdd(&px[np + pix], vf);
+    if (vs) atomicAdd(&px[2 * np + pix], vs);
+    if (vss) atomicAdd(&px[3 * np + pix], vss);
+    if (vbb) atomicAdd(&px[4 * np + pix], vbb);
+    if (ve) atomicAdd(&px[5 * np + pix], ve);
   }
 }
 
@@ -703,8 +839,10 @@ __global__ void k_probe_locate(DevScene S, int n, const float* p, int* out) {
 __global__ void k_probe_nu(DevScene S, int table, int order, int n, const float* x, float* out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  int dim = S.nu[table].dim;
-  out[i] = nu_interp(S, table, order, x + (size_t)i * dim, (S.compat & MONSEL_COMPAT_D3_ABSCISSAE) != 0);
+  const int dim = S.nu[table].dim;
+  double xd[4] = {0., 0., 0., 0.};
+  for (int k = 0; k < dim; ++k) xd[k] = x[(size_t)i * dim + k];
+  out[i] = (float)nu_interp(S, table, order, xd, (S.compat & MONSEL_COMPAT_D3_ABSCISSAE) != 0);
 }
 
 /* ===================================================================== */
@@ -724,8 +862,28 @@ const double kMe = 9.1093826e-31;
 struct DBuf {
   void* p = nullptr;
   size_t n = 0;
-  ~DBuf() {
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) {
+      reset();
+      p = o.p;
+      n = o.n;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
+  ~DBuf() { reset(); }
+  void reset() {
     if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
   }
   void ensure(size_t bytes) {
     if (bytes <= n) return;
@@ -754,6 +912,7 @@ struct HostScene {
   std::vector<float> spwem, x1;
   std::vector<DNUTable> nu;
   std::vector<float> nudata;
+  std::vector<double> nudata64;
   uint32_t compat = 0;
   float chamber_r = 1e8f;
   float chamber_c[3] = {0, 0, 0};
@@ -763,7 +922,7 @@ struct SceneDev {
   DBuf blob;
   DevScene dev{};
   size_t bytes = 0;
-  monsel_scene_desc copyOfCounts{};
+  HostScene host; /* kept to assemble batch scenes */
 };
 
 /* JoyLuoNieminenCSD::init (JoyLuoNieminenCSD.cu:42-82) in SI, then eV/nm. */
@@ -849,7 +1008,10 @@ HostScene compile_scene(const monsel_scene_desc* d) {
       if (t.n[k] < 2) throw MonselError(MONSEL_E_INVALID, "NU table axis too short");
       dt.n[k] = t.n[k];
       dt.xoff[k] = (int)H.nudata.size();
-      for (int j = 0; j < t.n[k]; ++j) H.nudata.push_back((float)t.x[k][j]);
+      for (int j = 0; j < t.n[k]; ++j) {
+        H.nudata.push_back((float)t.x[k][j]);
+        H.nudata64.push_back(t.x[k][j]);
+      }
       tot *= (size_t)t.n[k];
     }
     dt.voff = (int)H.nudata.size();
@@ -857,6 +1019,7 @@ HostScene compile_scene(const monsel_scene_desc* d) {
     for (size_t j = 0; j < tot; ++j) {
       double v = t.values[j];
       H.nudata.push_back((float)v);
+      H.nudata64.push_back(v);
       /* NUTableInterpolation.cu:169-172 */
       if (v < r0) r0 = v;
       else if (v > r1) r1 = v;
@@ -1001,6 +1164,26 @@ HostScene compile_scene(const monsel_scene_desc* d) {
           if (mat.n_core > kMaxCore) throw MonselError(MONSEL_E_UNSUPPORTED, "too many core levels");
           dm.p[TP_NCORE] = (float)mat.n_core;
           for (int c = 0; c < mat.n_core; ++c) dm.p[TP_CORE0 + c] = (float)mat.core_energy[c];
+          /* the same slots in J (FP64) for the scatter; TabulatedInelasticSM.cu:461-498 */
+          dm.q[TP_OFFSET] = e.p[0];
+          dm.q[TP_MINEGEN] = e.p[1];
+          dm.q[TP_RATEMULT] = e.p[2];
+          dm.q[TP_E0DISP] = e.p[3];
+          dm.q[TP_CB] = mat.energy_cb_bottom;
+          dm.q[TP_BANDGAP] = gap;
+          dm.q[TP_GAP] = gap;
+          dm.q[TP_OFFSETFERMI] = eFermi + e.p[0];
+          dm.q[TP_BEREF] = gap > 0. ? -gap : eFermi;
+          dm.q[TP_EIDOM0] = lo;
+          dm.q[TP_EIDOM1] = hi;
+          dm.q[TP_IIMFPDOM0] = dom0[e.table[0]][0];
+          dm.q[TP_IIMFPDOM1] = dom0[e.table[0]][1];
+          if (method >= 2) {
+            dm.q[TP_SE0RANGE0] = range[e.table[3]][0];
+            dm.q[TP_SE0RANGE1] = range[e.table[3]][1];
+          }
+          dm.q[TP_NCORE] = mat.n_core;
+          for (int c = 0; c < mat.n_core; ++c) dm.q[TP_CORE0 + c] = mat.core_energy[c];
           break;
         }
         default:
@@ -1125,6 +1308,7 @@ size_t host_scene_bytes(const HostScene& H) {
   add(H.x1.size() * sizeof(float));
   add(H.nu.size() * sizeof(DNUTable));
   add(H.nudata.size() * sizeof(float));
+  add(H.nudata64.size() * sizeof(double));
   return total;
 }
 
@@ -1134,6 +1318,7 @@ void upload_scene(const HostScene& H, SceneDev& sd) {
   size_t o_pr = append(blob, H.prims), o_pl = append(blob, H.planes), o_msm = append(blob, H.msms);
   size_t o_me = append(blob, H.mechs), o_el = append(blob, H.elems), o_sp = append(blob, H.spwem);
   size_t o_x1 = append(blob, H.x1), o_nu = append(blob, H.nu), o_nd = append(blob, H.nudata);
+  size_t o_nd64 = append(blob, H.nudata64);
   sd.blob.ensure(blob.size() + 256);
   HIP_TRY(hipMemcpy(sd.blob.p, blob.data(), blob.size(), hipMemcpyHostToDevice));
   char* b = sd.blob.as<char>();
@@ -1162,7 +1347,69 @@ void upload_scene(const HostScene& H, SceneDev& sd) {
   S.mott_x1 = (const float*)(b + o_x1);
   S.nu = (const DNUTable*)(b + o_nu);
   S.nu_data = (const float*)(b + o_nd);
+  S.nu_data64 = (const double*)(b + o_nd64);
   sd.bytes = blob.size();
+}
+
+/* One device scene holding several scenes side by side: every index is
+ * shifted by the arrays that precede it; roots[k] is the chamber region of
+ * part k.  Electrons never leave their own part, so a batch of jobs on
+ * different scenes runs in the same launches. */
+HostScene merge_scenes(const std::vector<const HostScene*>& parts, std::vector<int>& roots) {
+  HostScene M;
+  M.chamber_r = 0.f;
+  roots.clear();
+  for (const HostScene* P : parts) {
+    const int o_reg = (int)M.regions.size(), o_ch = (int)M.children.size(), o_sh = (int)M.shapes.size();
+    const int o_pr = (int)M.prims.size(), o_pl = (int)M.planes.size(), o_msm = (int)M.msms.size();
+    const int o_me = (int)M.mechs.size(), o_el = (int)M.elems.size(), o_mott = (int)(M.spwem.size() / kMottNE);
+    const int o_nu = (int)M.nu.size(), o_nd = (int)M.nudata.size();
+    roots.push_back(o_reg);
+    for (DRegion r : P->regions) {
+      if (r.parent >= 0) r.parent += o_reg;
+      r.shape += o_sh;
+      r.msm += o_msm;
+      r.first_child += o_ch;
+      M.regions.push_back(r);
+    }
+    for (int c : P->children) M.children.push_back(c + o_reg);
+    for (DShape sh : P->shapes) {
+      sh.first_prim += o_pr;
+      M.shapes.push_back(sh);
+    }
+    for (DPrim pr : P->prims) {
+      if (pr.kind == PRIM_PLANES) pr.first_plane += o_pl;
+      M.prims.push_back(pr);
+    }
+    M.planes.insert(M.planes.end(), P->planes.begin(), P->planes.end());
+    for (DMSM m : P->msms) {
+      m.mech0 += o_me;
+      M.msms.push_back(m);
+    }
+    for (DMech m : P->mechs) {
+      m.elem0 += o_el;
+      if (m.kind == MONSEL_MECH_TABULATED_INELASTIC)
+        for (int q = 0; q < 4; ++q)
+          if (m.table[q] >= 0) m.table[q] += o_nu;
+      M.mechs.push_back(m);
+    }
+    for (DElem el : P->elems) {
+      if (el.mott >= 0) el.mott += o_mott;
+      M.elems.push_back(el);
+    }
+    M.spwem.insert(M.spwem.end(), P->spwem.begin(), P->spwem.end());
+    M.x1.insert(M.x1.end(), P->x1.begin(), P->x1.end());
+    for (DNUTable t : P->nu) {
+      for (int k = 0; k < t.dim; ++k) t.xoff[k] += o_nd;
+      t.voff += o_nd;
+      M.nu.push_back(t);
+    }
+    M.nudata.insert(M.nudata.end(), P->nudata.begin(), P->nudata.end());
+    M.nudata64.insert(M.nudata64.end(), P->nudata64.begin(), P->nudata64.end());
+    M.compat = P->compat;
+    M.chamber_r = std::max(M.chamber_r, P->chamber_r);
+  }
+  return M;
 }
 
 /* ===================================================================== */
@@ -1182,7 +1429,16 @@ struct Job {
   int status = MONSEL_E_NOTREADY;
   std::string error;
   bool done = false;
-  std::chrono::steady_clock::time_point t_submit;
+  std::chrono::steady_clock::time_point t_submit, t_start;
+  /* engine state (worker thread only) */
+  std::shared_ptr<SceneDev> sd;
+  int slot = -1;
+  uint64_t total = 0, launched = 0;
+  uint32_t K = 0;             /* queue entries reserved per launched primary */
+  uint64_t reserved = 0;      /* queue entries reserved for its electrons in flight */
+  uint32_t outstanding = 0;   /* its electrons in the current queue */
+  int generations = 0;
+  DBuf prim_counts, prim_fwd, prim_energy, px, hist, stats_dev, centers_dev;
 };
 
 }  // namespace
@@ -1202,9 +1458,19 @@ struct monsel_ctx_s {
   uint64_t next_id = 1;
   bool stop = false;
   std::thread worker;
-  /* job workspace (worker thread only) */
-  DBuf q[2], esc, qcount, prim, pix, hist, stats, centers;
-  unsigned int* host_count = nullptr; /* pinned */
+  /* engine (worker thread only) */
+  DBuf q[2], esc, counts, segs, jobtab;
+  uint32_t qcap = 0;          /* entries per queue buffer */
+  int cur = 0;                /* queue holding the next launch's input */
+  uint32_t in_count = 0;      /* electrons in q[cur] */
+  unsigned int* host_count = nullptr; /* pinned: out, esc, per-job counts */
+  uint32_t* host_segs = nullptr;      /* pinned: segment staging */
+  std::vector<std::shared_ptr<Job>> active;
+  std::vector<int> free_slots;
+  JobRec jobrec[kMaxJobs];
+  SceneDev batch;             /* the scenes of the active jobs, side by side */
+  std::vector<std::shared_ptr<SceneDev>> batch_parts;
+  std::vector<int> batch_roots;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evk0 = nullptr, evk1 = nullptr;
   int blocks_per_cu = 0;
 };
@@ -1218,213 +1484,373 @@ uint32_t secondaries_per_primary(double E0eV) {
   return (uint32_t)k;
 }
 
-void run_job(monsel_ctx_s* C, Job& job, SceneDev& sd) {
+uint64_t splitmix64_host(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+constexpr size_t kQueueEntryBytes = (size_t)(kQF + kQU) * 4;
+constexpr size_t kEscEntryBytes = (size_t)(kEF + kEU) * 4;
+
+/* Queue entries for the device budget: both generation buffers plus the
+ * escape records (tuning.queue_capacity overrides). */
+uint32_t queue_capacity(monsel_ctx_s* C) {
+  if (C->tuning.queue_capacity > 0) return (uint32_t)C->tuning.queue_capacity;
+  size_t budget = std::min<size_t>((size_t)6 << 30, C->hbm / 8);
+  uint64_t cap = budget / (2 * kQueueEntryBytes + kEscEntryBytes);
+  return (uint32_t)std::min<uint64_t>(cap, 0x7FFFFFF0ull);
+}
+
+/* Rebuilds the batch scene from the scenes of the active jobs. Only between
+ * launches (the stream is idle). */
+void rebuild_batch_scene(monsel_ctx_s* C) {
+  std::vector<const HostScene*> parts;
+  for (auto& p : C->batch_parts) parts.push_back(&p->host);
+  HostScene M = merge_scenes(parts, C->batch_roots);
+  SceneDev fresh;
+  upload_scene(M, fresh);
+  C->batch.blob = std::move(fresh.blob);
+  C->batch.dev = fresh.dev;
+  C->batch.bytes = fresh.bytes;
+}
+
+void finish_job(monsel_ctx_s* C, const std::shared_ptr<Job>& job, int st, const std::string& msg) {
+  {
+    std::lock_guard<std::mutex> lk(C->mu);
+    job->status = st;
+    job->error = msg;
+    job->stats.total_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - job->t_submit).count();
+    job->done = true;
+  }
+  C->cv_done.notify_all();
+}
+
+/* Admits a job: slot, device buffers, its scene in the batch scene. */
+void setup_job(monsel_ctx_s* C, Job& job) {
   const monsel_job_desc& jd = job.desc;
-  const uint64_t total = (uint64_t)jd.n_pixels * (uint64_t)jd.primaries_per_pixel;
   const int np = jd.n_pixels;
   const int nbins = jd.n_energy_bins > 0 ? jd.n_energy_bins : 1;
   const double E0eV = jd.beam_energy / kE;
-  /* pixel tallies + histograms */
-  C->pix.ensure((size_t)np * 6 * 8);
-  unsigned long long* back_px = C->pix.as<unsigned long long>();
-  unsigned long long* fwd_px = back_px + np;
-  unsigned long long* se_px = fwd_px + np;
-  unsigned long long* se_sq = se_px + np;
-  unsigned long long* bse_sq = se_sq + np;
-  double* benergy = (double*)(bse_sq + np);
-  size_t hn = (size_t)2 * (nbins + 2) + 182 + 362;
-  C->hist.ensure(hn * 8);
-  unsigned long long* hback = C->hist.as<unsigned long long>();
-  unsigned long long* hfwd = hback + nbins + 2;
-  unsigned long long* helev = hfwd + nbins + 2;
-  unsigned long long* hazim = helev + 182;
-  C->stats.ensure(16 * 8);
-  C->qcount.ensure(4 * 256);
-  C->centers.ensure(sizeof(double) * 3 * (size_t)np);
-  HIP_TRY(hipMemcpyAsync(C->centers.p, job.centers.data(), sizeof(double) * 3 * (size_t)np, hipMemcpyHostToDevice,
-                         C->stream));
-  HIP_TRY(hipMemsetAsync(C->pix.p, 0, (size_t)np * 6 * 8, C->stream));
-  HIP_TRY(hipMemsetAsync(C->hist.p, 0, hn * 8, C->stream));
-  HIP_TRY(hipMemsetAsync(C->stats.p, 0, 16 * 8, C->stream));
-  /* chunking: queue memory per chunk bounded */
-  const uint32_t K = secondaries_per_primary(E0eV);
-  const size_t perEntry = (size_t)(kQF + kQU) * 4;
-  const size_t perEsc = (size_t)(kEF + kEU) * 4;
-  const size_t budget = (size_t)6 << 30; /* both generation buffers + escape records */
-  uint64_t chunkMax = budget / ((2 * perEntry + perEsc) * K);
-  if (C->tuning.queue_capacity > 0) chunkMax = std::max<uint64_t>(1, (uint64_t)C->tuning.queue_capacity / K);
-  if (chunkMax < 1024) chunkMax = 1024;
-  int block = C->tuning.block_size > 0 ? C->tuning.block_size : 256;
-  int grid = C->n_cu * (C->blocks_per_cu > 0 ? C->blocks_per_cu : 8);
-  int maxGen = C->tuning.max_generations > 0 ? C->tuning.max_generations : 64;
-  uint32_t k0 = (uint32_t)jd.seed, k1 = (uint32_t)(jd.seed >> 32);
-  /* Event markers time the kernels; MONSEL_NO_EVENT_TIMING=1 leaves them out
-   * (hardware-counter profiling sessions reject the marker packets). */
-  const bool timing = !getenv("MONSEL_NO_EVENT_TIMING");
-  if (timing) HIP_TRY(hipEventRecord(C->ev0, C->stream));
-  int generations = 0;
-  for (uint64_t start = 0; start < total; start += chunkMax) {
-    uint32_t n = (uint32_t)std::min<uint64_t>(chunkMax, total - start);
-    uint32_t cap = (uint32_t)std::min<uint64_t>((uint64_t)n * K, 0xFFFFFFF0ull);
-    C->q[0].ensure((size_t)cap * perEntry);
-    C->q[1].ensure((size_t)cap * perEntry);
-    C->esc.ensure((size_t)cap * perEsc);
-    C->prim.ensure((size_t)n * 4);
-    HIP_TRY(hipMemsetAsync(C->prim.p, 0, (size_t)n * 4, C->stream));
-    JobDev J;
-    memset(&J, 0, sizeof(J));
-    J.k0 = k0;
-    J.k1 = k1;
-    J.E0 = (float)E0eV;
-    J.width = (float)(jd.beam_width * 1e9);
-    J.centers = nullptr;
-    J.n_pixels = np;
-    J.ppp = jd.primaries_per_pixel;
-    J.prim_base = (uint32_t)(jd.primary_offset + start);
-    J.chunk_start = (uint32_t)start;
-    J.chunk_n = n;
-    J.nbins = nbins;
-    J.E0eV = E0eV;
-    J.maxSEbin = (int)(jd.se_cutoff_ev / (E0eV / nbins));
-    J.max_steps = jd.max_steps > 0 ? jd.max_steps : 1000000;
-    J.prim_counts = C->prim.as<uint32_t>();
-    J.fwd_px = fwd_px;
-    J.back_energy_px = benergy;
-    J.hist_back = hback;
-    J.hist_fwd = hfwd;
-    J.hist_elev = helev;
-    J.hist_azim = hazim;
-    J.stats = C->stats.as<unsigned long long>();
-    J.centers = C->centers.as<double>();
-    unsigned int* counts = C->qcount.as<unsigned int>();
-    /* counts[0]: claim counter, counts[1]: spawned secondaries, counts[2]: escapes */
-    uint32_t in_count = n;
-    int cur = 0;
-    for (int g = 0; g < maxGen && in_count > 0; ++g) {
-      HIP_TRY(hipMemsetAsync(counts, 0, 3 * sizeof(unsigned int), C->stream));
-      GenDev G;
-      memset(&G, 0, sizeof(G));
-      G.mode = g == 0 ? 0 : 1;
-      G.in_f = C->q[cur].as<float>();
-      G.in_u = (const uint32_t*)(C->q[cur].as<float>() + (size_t)kQF * cap);
-      G.in_cap = cap;
-      G.in_n = in_count;
-      G.claim = counts;
-      G.out_f = C->q[cur ^ 1].as<float>();
-      G.out_u = (uint32_t*)(C->q[cur ^ 1].as<float>() + (size_t)kQF * cap);
-      G.out_cap = cap;
-      G.out_count = counts + 1;
-      G.esc_f = C->esc.as<float>();
-      G.esc_u = (uint32_t*)(C->esc.as<float>() + (size_t)kEF * cap);
-      G.esc_cap = cap;
-      G.esc_count = counts + 2;
-      if (timing) HIP_TRY(hipEventRecord(C->evk0, C->stream));
-      g_hip_call = "launch k_transport";
-      hipLaunchKernelGGL(k_transport, dim3(grid), dim3(block), 0, C->stream, sd.dev, J, G);
-      HIP_TRY(hipGetLastError());
-      if (timing) HIP_TRY(hipEventRecord(C->evk1, C->stream));
-      HIP_TRY(hipMemcpyAsync(C->host_count, counts + 1, sizeof(unsigned int), hipMemcpyDeviceToHost, C->stream));
-      g_hip_call = "launch k_tally_escapes";
-      hipLaunchKernelGGL(k_tally_escapes, dim3(C->n_cu * 4), dim3(256), 0, C->stream, sd.dev, J, G.esc_f, G.esc_u,
-                         cap, (const unsigned int*)(counts + 2));
-      HIP_TRY(hipGetLastError());
-      HIP_TRY(hipStreamSynchronize(C->stream));
-      uint32_t outc = *C->host_count;
-      in_count = outc > cap ? cap : outc;
-      float kms = 0.f;
-      if (timing) HIP_TRY(hipEventElapsedTime(&kms, C->evk0, C->evk1));
-      job.stats.transport_ms += kms;
-      job.stats.transport_launches += 1;
-      cur ^= 1;
-      generations = std::max(generations, g + 1);
-    }
-    /* generation limit: any electrons left are dropped and counted */
-    if (in_count > 0) job.stats.queue_overflow += in_count;
-    int rb = 256;
+  job.total = (uint64_t)jd.n_pixels * (uint64_t)jd.primaries_per_pixel;
+  job.K = secondaries_per_primary(E0eV);
+  job.launched = job.reserved = 0;
+  job.outstanding = 0;
+  job.generations = 0;
+  job.t_start = std::chrono::steady_clock::now();
+  int part = -1;
+  for (size_t k = 0; k < C->batch_parts.size(); ++k)
+    if (C->batch_parts[k] == job.sd) part = (int)k;
+  if (part < 0) {
+    C->batch_parts.push_back(job.sd);
+    rebuild_batch_scene(C);
+    part = (int)C->batch_parts.size() - 1;
+  }
+  job.slot = C->free_slots.back();
+  C->free_slots.pop_back();
+  const size_t n = (size_t)std::max<uint64_t>(job.total, 1);
+  job.prim_counts.ensure(n * 4);
+  job.prim_fwd.ensure(n * 4);
+  job.prim_energy.ensure(n * 8);
+  job.px.ensure((size_t)np * 6 * 8);
+  job.hist.ensure((size_t)hist_len(nbins) * 8);
+  job.stats_dev.ensure(8 * 8);
+  job.centers_dev.ensure((size_t)np * 3 * 8);
+  HIP_TRY(hipMemsetAsync(job.prim_counts.p, 0, n * 4, C->stream));
+  HIP_TRY(hipMemsetAsync(job.prim_fwd.p, 0, n * 4, C->stream));
+  HIP_TRY(hipMemsetAsync(job.prim_energy.p, 0, n * 8, C->stream));
+  HIP_TRY(hipMemsetAsync(job.px.p, 0, (size_t)np * 6 * 8, C->stream));
+  HIP_TRY(hipMemsetAsync(job.hist.p, 0, (size_t)hist_len(nbins) * 8, C->stream));
+  HIP_TRY(hipMemsetAsync(job.stats_dev.p, 0, 8 * 8, C->stream));
+  HIP_TRY(hipMemcpyAsync(job.centers_dev.p, job.centers.data(), (size_t)np * 3 * 8, hipMemcpyHostToDevice, C->stream));
+  const HostScene& H = job.sd->host;
+  JobRec& R = C->jobrec[job.slot];
+  memset(&R, 0, sizeof(R));
+  R.lin0 = splitmix64_host(jd.seed ^ 0x5EED5EED5EED5EEDull);
+  R.E0 = (float)E0eV;
+  R.width = (float)(jd.beam_width * 1e9);
+  R.centers = job.centers_dev.as<double>();
+  R.n_pixels = np;
+  R.ppp = jd.primaries_per_pixel;
+  R.prim_base = (uint32_t)jd.primary_offset;
+  R.root = C->batch_roots[part];
+  R.max_steps = jd.max_steps > 0 ? jd.max_steps : 1000000;
+  R.nbins = nbins;
+  R.maxSEbin = (int)(jd.se_cutoff_ev / (E0eV / nbins));
+  R.killed = 0;
+  R.E0eV = E0eV;
+  for (int k = 0; k < 3; ++k) R.chamber_c[k] = H.chamber_c[k];
+  R.chamber_r = H.chamber_r;
+  R.prim_counts = job.prim_counts.as<uint32_t>();
+  R.prim_fwd = job.prim_fwd.as<uint32_t>();
+  R.prim_energy = job.prim_energy.as<unsigned long long>();
+  R.hist = job.hist.as<unsigned long long>();
+  R.stats = job.stats_dev.as<unsigned long long>();
+  HIP_TRY(hipMemcpyAsync(C->jobtab.as<JobRec>() + job.slot, &R, sizeof(JobRec), hipMemcpyHostToDevice, C->stream));
+  HIP_TRY(hipStreamSynchronize(C->stream));
+}
+
+/* Per-pixel reduction and read-back of a job whose electrons are all done. */
+void complete_job(monsel_ctx_s* C, Job& job) {
+  const monsel_job_desc& jd = job.desc;
+  const int np = jd.n_pixels;
+  const int nbins = C->jobrec[job.slot].nbins;
+  if (job.total > 0) {
+    const int rb = 256;
     g_hip_call = "launch k_reduce_primaries";
-    hipLaunchKernelGGL(k_reduce_primaries, dim3((n + rb - 1) / rb), dim3(rb), 0, C->stream, J, back_px, se_px, se_sq,
-                       bse_sq);
+    hipLaunchKernelGGL(k_reduce_primaries, dim3((unsigned)((job.total + rb - 1) / rb)), dim3(rb), 0, C->stream,
+                       (const JobRec*)C->jobtab.as<JobRec>(), job.slot, (uint32_t)job.total,
+                       job.px.as<unsigned long long>());
     HIP_TRY(hipGetLastError());
   }
-  if (timing) HIP_TRY(hipEventRecord(C->ev1, C->stream));
-  /* results */
-  job.back.resize(np);
-  job.fwd.resize(np);
-  job.se.resize(np);
+  std::vector<unsigned long long> px((size_t)np * 6), h((size_t)hist_len(nbins)), st(8);
+  HIP_TRY(hipMemcpyAsync(px.data(), job.px.p, px.size() * 8, hipMemcpyDeviceToHost, C->stream));
+  HIP_TRY(hipMemcpyAsync(h.data(), job.hist.p, h.size() * 8, hipMemcpyDeviceToHost, C->stream));
+  HIP_TRY(hipMemcpyAsync(st.data(), job.stats_dev.p, 8 * 8, hipMemcpyDeviceToHost, C->stream));
+  HIP_TRY(hipStreamSynchronize(C->stream));
+  job.back.assign(px.begin(), px.begin() + np);
+  job.fwd.assign(px.begin() + np, px.begin() + 2 * np);
+  job.se.assign(px.begin() + 2 * np, px.begin() + 3 * np);
   job.se_sq.resize(np);
   job.bse_sq.resize(np);
   job.back_energy.resize(np);
-  job.hback.resize(nbins + 2);
-  job.hfwd.resize(nbins + 2);
-  job.helev.resize(182);
-  job.hazim.resize(362);
-  std::vector<unsigned long long> st(16), sq(2 * (size_t)np);
-  HIP_TRY(hipMemcpyAsync(job.back.data(), back_px, 8 * np, hipMemcpyDeviceToHost, C->stream));
-  HIP_TRY(hipMemcpyAsync(job.fwd.data(), fwd_px, 8 * np, hipMemcpyDeviceToHost, C->stream));
-  HIP_TRY(hipMemcpyAsync(job.se.data(), se_px, 8 * np, hipMemcpyDeviceToHost, C->stream));
-  HIP_TRY(hipMemcpyAsync(sq.data(), se_sq, 16 * (size_t)np, hipMemcpyDeviceToHost, C->stream));
-  HIP_TRY(hipMemcpyAsync(job.back_energy.data(), benergy, 8 * np, hipMemcpyDeviceToHost, C->stream));
-  HIP_TRY(hipMemcpyAsync(job.hback.data(), hback, 8 * (nbins + 2), hipMemcpyDeviceToHost, C->stream));
-  HIP_TRY(hipMemcpyAsync(job.hfwd.data(), hfwd, 8 * (nbins + 2), hipMemcpyDeviceToHost, C->stream));
-  HIP_TRY(hipMemcpyAsync(job.helev.data(), helev, 8 * 182, hipMemcpyDeviceToHost, C->stream));
-  HIP_TRY(hipMemcpyAsync(job.hazim.data(), hazim, 8 * 362, hipMemcpyDeviceToHost, C->stream));
-  HIP_TRY(hipMemcpyAsync(st.data(), C->stats.p, 8 * 16, hipMemcpyDeviceToHost, C->stream));
-  HIP_TRY(hipStreamSynchronize(C->stream));
-  for (size_t p = 0; p < (size_t)np; ++p) {
-    job.se_sq[p] = (double)sq[p];
-    job.bse_sq[p] = (double)sq[np + p];
+  for (int p = 0; p < np; ++p) {
+    job.se_sq[p] = (double)px[3 * (size_t)np + p];
+    job.bse_sq[p] = (double)px[4 * (size_t)np + p];
+    job.back_energy[p] = (double)px[5 * (size_t)np + p] / kEnergyScale;
   }
-  float ms = 0.f;
-  if (timing) HIP_TRY(hipEventElapsedTime(&ms, C->ev0, C->ev1));
-  job.stats.primaries = total;
+  const int nb2 = nbins + 2;
+  job.hback.assign(h.begin(), h.begin() + nb2);
+  job.hfwd.assign(h.begin() + nb2, h.begin() + 2 * nb2);
+  job.helev.assign(h.begin() + 2 * nb2, h.begin() + 2 * nb2 + 182);
+  job.hazim.assign(h.begin() + 2 * nb2 + 182, h.begin() + 2 * nb2 + 182 + 362);
+  job.stats.primaries = job.total;
   job.stats.steps = st[kStatSteps];
   job.stats.electrons = st[kStatElectrons];
   job.stats.escapes = st[kStatEscapes];
   job.stats.watchdog_drops = st[kStatWatchdog];
-  job.stats.queue_overflow += st[kStatOverflow];
-  job.stats.generations = generations;
-  job.stats.device_ms = ms;
+  job.stats.queue_overflow = st[kStatOverflow];
+  job.stats.generations = job.generations;
+  job.stats.device_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - job.t_start).count();
+}
+
+void release_job(monsel_ctx_s* C, Job& job) {
+  C->free_slots.push_back(job.slot);
+  job.slot = -1;
+  job.prim_counts.reset();
+  job.prim_fwd.reset();
+  job.prim_energy.reset();
+  job.px.reset();
+  job.hist.reset();
+  job.stats_dev.reset();
+  job.centers_dev.reset();
+  job.sd.reset();
+}
+
+/* One launch of the transport kernel over new primaries of the active jobs
+ * and every electron queued by the previous launch, then its tallies. */
+void run_generation(monsel_ctx_s* C) {
+  const uint32_t cap = C->qcap;
+  const int maxGen = C->tuning.max_generations > 0 ? C->tuning.max_generations : 100000;
+  const uint64_t perLaunch = C->tuning.primaries_per_launch > 0 ? (uint64_t)C->tuning.primaries_per_launch : cap;
+  /* jobs with electrons in flight before this launch */
+  std::vector<Job*> inflight;
+  uint64_t reservedTotal = 0;
+  for (auto& j : C->active) reservedTotal += j->reserved;
+  /* new primaries: FIFO over the active jobs, within the queue reservation */
+  int nseg = 0;
+  uint64_t nprim = 0;
+  for (auto& jp : C->active) {
+    Job& j = *jp;
+    if (j.launched >= j.total || nseg >= kMaxJobs) continue;
+    const uint64_t avail = cap > reservedTotal ? cap - reservedTotal : 0;
+    uint64_t n = std::min<uint64_t>(j.total - j.launched, avail / j.K);
+    n = std::min<uint64_t>(n, perLaunch - nprim);
+    if (n == 0) continue;
+    C->host_segs[nseg] = (uint32_t)(nprim + n);
+    C->host_segs[kMaxJobs + nseg] = (uint32_t)j.slot;
+    C->host_segs[2 * kMaxJobs + nseg] = (uint32_t)j.launched;
+    ++nseg;
+    nprim += n;
+    j.launched += n;
+    j.reserved += n * j.K;
+    reservedTotal += n * j.K;
+  }
+  for (auto& jp : C->active)
+    if (jp->outstanding > 0 || jp->reserved > 0) inflight.push_back(jp.get());
+  if (nprim == 0 && C->in_count == 0) {
+    /* nothing to run: jobs without primaries complete; a job whose primaries
+     * cannot be admitted into an empty queue is too large for it */
+    for (auto& jp : C->active)
+      if (jp->launched < jp->total) throw MonselError(MONSEL_E_NOMEM, "queue capacity below one primary's reservation");
+    return;
+  }
+  unsigned int* counts = C->counts.as<unsigned int>();
+  /* counts[0] claim, [1] queued electrons, [2] escapes, [4..] per-job queue counts */
+  HIP_TRY(hipMemsetAsync(counts, 0, (4 + kMaxJobs) * sizeof(unsigned int), C->stream));
+  if (nseg > 0)
+    HIP_TRY(hipMemcpyAsync(C->segs.p, C->host_segs, 3 * kMaxJobs * 4, hipMemcpyHostToDevice, C->stream));
+  const int cur = C->cur;
+  GenDev G;
+  memset(&G, 0, sizeof(G));
+  G.jobs = C->jobtab.as<JobRec>();
+  G.seg_end = C->segs.as<uint32_t>();
+  G.seg_job = C->segs.as<uint32_t>() + kMaxJobs;
+  G.seg_first = C->segs.as<uint32_t>() + 2 * kMaxJobs;
+  G.n_seg = nseg;
+  G.n_prim = (uint32_t)nprim;
+  G.in_f = C->q[cur].as<float>();
+  G.in_u = (const uint32_t*)(C->q[cur].as<float>() + (size_t)kQF * cap);
+  G.in_cap = cap;
+  G.in_n = C->in_count;
+  G.claim = counts;
+  G.out_f = C->q[cur ^ 1].as<float>();
+  G.out_u = (uint32_t*)(C->q[cur ^ 1].as<float>() + (size_t)kQF * cap);
+  G.out_cap = cap;
+  G.out_count = counts + 1;
+  G.esc_f = C->esc.as<float>();
+  G.esc_u = (uint32_t*)(C->esc.as<float>() + (size_t)kEF * cap);
+  G.esc_cap = cap;
+  G.esc_count = counts + 2;
+  G.steps_per_launch = C->tuning.steps_per_launch;
+  const int block = C->tuning.block_size > 0 ? C->tuning.block_size : 256;
+  const int grid = C->n_cu * (C->blocks_per_cu > 0 ? C->blocks_per_cu : 4);
+  /* Event markers time the kernels; MONSEL_NO_EVENT_TIMING=1 leaves them out
+   * (hardware-counter profiling sessions reject the marker packets). */
+  const bool timing = !getenv("MONSEL_NO_EVENT_TIMING");
+  if (timing) HIP_TRY(hipEventRecord(C->evk0, C->stream));
+  g_hip_call = "launch k_transport";
+  hipLaunchKernelGGL(k_transport, dim3(grid), dim3(block), 0, C->stream, C->batch.dev, G);
+  HIP_TRY(hipGetLastError());
+  if (timing) HIP_TRY(hipEventRecord(C->evk1, C->stream));
+  /* escape tallies: the histograms go through LDS when one job is in flight */
+  int lds_job = -1;
+  size_t lds_bytes = 0;
+  if (inflight.size() == 1) {
+    const int nb = C->jobrec[inflight[0]->slot].nbins;
+    lds_bytes = (size_t)hist_len(nb) * 4;
+    if (lds_bytes <= 64 * 1024) lds_job = inflight[0]->slot;
+    else lds_bytes = 0;
+  }
+  g_hip_call = "launch k_tally_escapes";
+  hipLaunchKernelGGL(k_tally_escapes, dim3(C->n_cu * 4), dim3(256), lds_bytes, C->stream, C->batch.dev,
+                     (const JobRec*)G.jobs, (const float*)G.esc_f, (const uint32_t*)G.esc_u, cap,
+                     (const unsigned int*)(counts + 2), lds_job);
+  HIP_TRY(hipGetLastError());
+  int n_slots = 0;
+  for (Job* j : inflight) n_slots = std::max(n_slots, j->slot + 1);
+  if (inflight.size() > 1) {
+    g_hip_call = "launch k_count_jobs";
+    hipLaunchKernelGGL(k_count_jobs, dim3(C->n_cu * 2), dim3(256), 0, C->stream,
+                       (const uint32_t*)(G.out_u + 4 * (size_t)cap), (const unsigned int*)(counts + 1), cap,
+                       counts + 4, n_slots);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipMemcpyAsync(C->host_count, counts, (4 + std::max(n_slots, 1)) * sizeof(unsigned int),
+                         hipMemcpyDeviceToHost, C->stream));
+  HIP_TRY(hipStreamSynchronize(C->stream));
+  const uint32_t outc = C->host_count[1];
+  C->in_count = outc > cap ? cap : outc;
+  C->cur = cur ^ 1;
+  float kms = 0.f;
+  if (timing) HIP_TRY(hipEventElapsedTime(&kms, C->evk0, C->evk1));
+  for (Job* j : inflight) {
+    j->outstanding = inflight.size() > 1 ? C->host_count[4 + j->slot] : C->in_count;
+    j->generations += 1;
+    j->stats.transport_ms += kms;
+    j->stats.transport_launches += 1;
+    if (j->outstanding == 0) j->reserved = 0;
+    if (j->generations > maxGen && !C->jobrec[j->slot].killed) {
+      /* generation limit: the job's remaining electrons are dropped and counted */
+      C->jobrec[j->slot].killed = 1;
+      HIP_TRY(hipMemcpyAsync(&C->jobtab.as<JobRec>()[j->slot].killed, &C->jobrec[j->slot].killed, sizeof(int),
+                             hipMemcpyHostToDevice, C->stream));
+      HIP_TRY(hipStreamSynchronize(C->stream));
+    }
+  }
 }
 
 void worker_main(monsel_ctx_s* C) {
   (void)hipSetDevice(C->device);
+  const int maxActive = std::min(kMaxJobs, C->tuning.max_jobs_in_flight > 0 ? C->tuning.max_jobs_in_flight : kMaxJobs);
   for (;;) {
-    std::shared_ptr<Job> job;
-    std::shared_ptr<SceneDev> sd;
+    std::vector<std::shared_ptr<Job>> admitted;
     {
       std::unique_lock<std::mutex> lk(C->mu);
-      C->cv_work.wait(lk, [&] { return C->stop || !C->queue.empty(); });
-      if (C->stop && C->queue.empty()) return;
-      job = C->queue.front();
-      C->queue.pop_front();
-      auto it = C->scenes.find(job->scene);
-      if (it != C->scenes.end()) sd = it->second;
-    }
-    int st = MONSEL_OK;
-    std::string msg;
-    if (!sd) {
-      st = MONSEL_E_NOTFOUND;
-      msg = "scene released before the job ran";
-    } else {
-      try {
-        run_job(C, *job, *sd);
-      } catch (const MonselError& e) {
-        st = e.status;
-        msg = e.msg;
-      } catch (const std::exception& e) {
-        st = MONSEL_E_DEVICE;
-        msg = std::string(e.what()) + " (in " + g_hip_call + ")";
+      if (C->active.empty())
+        C->cv_work.wait(lk, [&] { return C->stop || !C->queue.empty(); });
+      if (C->stop && C->queue.empty() && C->active.empty()) return;
+      while (!C->queue.empty() && (int)(C->active.size() + admitted.size()) < maxActive) {
+        auto j = C->queue.front();
+        auto it = C->scenes.find(j->scene);
+        if (it == C->scenes.end()) {
+          C->queue.pop_front();
+          lk.unlock();
+          finish_job(C, j, MONSEL_E_NOTFOUND, "scene released before the job ran");
+          lk.lock();
+          continue;
+        }
+        /* one batch scene holds scenes of one compatibility mode: a job of
+         * another mode waits until the jobs in flight are done */
+        const uint32_t mode = it->second->host.compat;
+        if (!admitted.empty() && admitted[0]->sd->host.compat != mode) break;
+        if (admitted.empty() && !C->active.empty() && C->active[0]->sd->host.compat != mode) break;
+        j->sd = it->second;
+        C->queue.pop_front();
+        admitted.push_back(j);
       }
     }
-    {
-      std::lock_guard<std::mutex> lk(C->mu);
-      job->status = st;
-      job->error = msg;
-      job->stats.total_ms =
-          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - job->t_submit).count();
-      job->done = true;
+    try {
+      if (C->active.empty() && C->in_count == 0 && !C->batch_parts.empty()) {
+        /* idle engine: start a fresh batch scene */
+        C->batch_parts.clear();
+        C->batch.blob.reset();
+        C->batch.dev = DevScene{};
+      }
+      for (auto& j : admitted) {
+        try {
+          setup_job(C, *j);
+          C->active.push_back(j);
+        } catch (const MonselError& e) {
+          if (j->slot >= 0) release_job(C, *j);
+          finish_job(C, j, e.status, e.msg);
+        }
+      }
+      if (C->active.empty()) continue;
+      run_generation(C);
+      /* completed jobs: all primaries launched and none of their electrons queued */
+      for (size_t k = 0; k < C->active.size();) {
+        auto j = C->active[k];
+        if (j->launched >= j->total && j->outstanding == 0) {
+          complete_job(C, *j);
+          release_job(C, *j);
+          C->active.erase(C->active.begin() + k);
+          finish_job(C, j, MONSEL_OK, "");
+        } else {
+          ++k;
+        }
+      }
+    } catch (const std::exception& ex) {
+      /* a device failure ends every job in flight */
+      std::string msg;
+      int st = MONSEL_E_DEVICE;
+      if (auto* me = dynamic_cast<const MonselError*>(&ex)) {
+        msg = me->msg;
+        st = me->status;
+      } else {
+        msg = std::string(ex.what()) + " (in " + g_hip_call + ")";
+      }
+      (void)hipStreamSynchronize(C->stream);
+      for (auto& j : C->active) {
+        release_job(C, *j);
+        finish_job(C, j, st, msg);
+      }
+      C->active.clear();
+      C->in_count = 0;
     }
-    C->cv_done.notify_all();
   }
 }
 
@@ -1473,7 +1899,20 @@ int monsel_create(int device_ordinal, const monsel_tuning* tuning, monsel_ctx* o
     C->hbm = prop.totalGlobalMem;
     C->name = prop.gcnArchName;
     HIP_TRY(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
-    HIP_TRY(hipHostMalloc((void**)&C->host_count, 64, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void**)&C->host_count, (4 + kMaxJobs) * sizeof(unsigned int), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void**)&C->host_segs, 3 * kMaxJobs * sizeof(uint32_t), hipHostMallocDefault));
+    if (C->tuning.max_jobs_in_flight < 0 || C->tuning.primaries_per_launch < 0 || C->tuning.steps_per_launch < 0 ||
+        C->tuning.queue_capacity < 0)
+      throw MonselError(MONSEL_E_INVALID, "negative tuning value");
+    C->jobtab.ensure(kMaxJobs * sizeof(JobRec));
+    C->counts.ensure((4 + kMaxJobs) * sizeof(unsigned int));
+    C->segs.ensure(3 * kMaxJobs * sizeof(uint32_t));
+    for (int k = kMaxJobs - 1; k >= 0; --k) C->free_slots.push_back(k);
+    C->qcap = queue_capacity(C.get());
+    if (C->qcap < 65536) throw MonselError(MONSEL_E_INVALID, "queue_capacity below 65536 entries");
+    C->q[0].ensure((size_t)C->qcap * kQueueEntryBytes);
+    C->q[1].ensure((size_t)C->qcap * kQueueEntryBytes);
+    C->esc.ensure((size_t)C->qcap * kEscEntryBytes);
     HIP_TRY(hipEventCreate(&C->ev0));
     HIP_TRY(hipEventCreate(&C->ev1));
     HIP_TRY(hipEventCreate(&C->evk0));
@@ -1507,6 +1946,7 @@ int monsel_destroy(monsel_ctx ctx) {
   ctx->scenes.clear();
   ctx->jobs.clear();
   if (ctx->host_count) (void)hipHostFree(ctx->host_count);
+  if (ctx->host_segs) (void)hipHostFree(ctx->host_segs);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->evk0) (void)hipEventDestroy(ctx->evk0);
@@ -1543,6 +1983,7 @@ int monsel_scene_upload(monsel_ctx ctx, const monsel_scene_desc* desc, monsel_sc
     auto sd = std::make_shared<SceneDev>();
     HIP_TRY(hipSetDevice(ctx->device));
     upload_scene(H, *sd);
+    sd->host = std::move(H);
     std::lock_guard<std::mutex> lk(ctx->mu);
     monsel_scene_id id = ctx->next_id++;
     ctx->scenes[id] = sd;
@@ -1560,13 +2001,14 @@ int monsel_scene_release(monsel_ctx ctx, monsel_scene_id scene) {
 
 int monsel_job_footprint(monsel_ctx ctx, const monsel_job_desc* job, size_t* hbm_bytes) {
   if (!ctx || !job || !hbm_bytes) return set_err(MONSEL_E_INVALID, "NULL argument");
-  uint64_t total = (uint64_t)job->n_pixels * (uint64_t)job->primaries_per_pixel;
-  uint32_t K = secondaries_per_primary(job->beam_energy / kE);
-  size_t perEntry = (size_t)(kQF + kQU) * 4;
-  uint64_t chunkMax = ((size_t)6 << 30) / (2 * perEntry * K);
-  uint64_t chunk = std::min<uint64_t>(total, chunkMax);
-  *hbm_bytes = (size_t)(2 * chunk * K * perEntry + chunk * 4 + (size_t)job->n_pixels * 72 +
-                        (size_t)(2 * (job->n_energy_bins + 2) + 544) * 8);
+  /* per-job buffers (the generation queues belong to the context: their
+   * share is the job's queue reservation) */
+  const uint64_t total = (uint64_t)job->n_pixels * (uint64_t)job->primaries_per_pixel;
+  const uint32_t K = secondaries_per_primary(job->beam_energy / kE);
+  const uint64_t resv = std::min<uint64_t>(total * K, ctx->qcap);
+  const int nbins = job->n_energy_bins > 0 ? job->n_energy_bins : 1;
+  *hbm_bytes = (size_t)(total * 16 + (uint64_t)job->n_pixels * (6 * 8 + 24) + (uint64_t)hist_len(nbins) * 8 + 64 +
+                        resv * (2 * kQueueEntryBytes + kEscEntryBytes));
   return MONSEL_OK;
 }
 

@@ -40,7 +40,9 @@ constexpr float kEcharge = 1.60217653e-19f;         /* J per eV */
 /* ----------------------------------------------------------- RNG ------- */
 
 /* Philox4x32-10 (Salmon et al., SC'11).  Counter words: (block, lineage lo,
- * lineage hi, primary index); key: the job seed. */
+ * lineage hi, primary index) under one fixed key; a job's seed enters
+ * through the lineage of its primaries (splitmix64 of the seed), a
+ * secondary's lineage is derived from its parent's. */
 MD void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
                       uint32_t& o0, uint32_t& o1, uint32_t& o2, uint32_t& o3) {
 #pragma unroll
@@ -82,6 +84,7 @@ struct Elec {
   int region;
   uint32_t primary;
   uint32_t lin_lo, lin_hi;
+  uint32_t job;
   uint32_t nchild;
   uint32_t steps;
 };
@@ -110,6 +113,7 @@ MD void step_uniforms(const Elec& e, uint32_t k0, uint32_t k1, float* u) {
 
 /* block reserved for the primary's initial position (never a step block) */
 constexpr uint32_t kPrimaryBlock = 0xFFFFFFFFu;
+constexpr uint32_t kPhiloxKey0 = 0x6D6F6E73u, kPhiloxKey1 = 0x656C3235u;
 
 /* Lanes per block of every kernel that uses the per-lane LDS arrays below. */
 constexpr int kLaneBlock = 256;
@@ -303,9 +307,23 @@ MD float csd_compute(const DCSD& c, float len, float kE) {
 
 /* ------------------------------------------------------- NU tables ----- */
 
+/* Non-uniform table interpolation, generic over the arithmetic type T:
+ * float for the inelastic mean free path (a well-conditioned 1-D lookup),
+ * double for the tabulated scatter, whose 3-D angle lookup in reference
+ * mode extrapolates a cubic far outside its abscissae
+ * (NULagrangeInterpolation.cu:236) with weights up to ~1e7, so that FP32
+ * rounding of the table values would change the result. */
+template <typename T>
+MD const T* nu_base(const DevScene& S);
+template <>
+MD const float* nu_base<float>(const DevScene& S) { return S.nu_data; }
+template <>
+MD const double* nu_base<double>(const DevScene& S) { return S.nu_data64; }
+
 /* NULagrangeInterpolation::locate (NULagrangeInterpolation.cu:5-68) on an
  * ascending or descending grid: first index of the order+1 window. */
-MD int nu_locate(const float* xs, int n, int order, float x, int& nearest) {
+template <typename T>
+MD int nu_locate(const T* xs, int n, int order, T x) {
   int lowlim = -1, uplim = n, maxindex = n - 1;
   bool asc = xs[maxindex] > xs[0];
   while (uplim - lowlim > 1) {
@@ -313,20 +331,12 @@ MD int nu_locate(const float* xs, int n, int order, float x, int& nearest) {
     if ((x > xs[mid]) == asc) lowlim = mid;
     else uplim = mid;
   }
-  if (lowlim < 0) {
-    nearest = 0;
-    return 0;
-  }
-  if (uplim > maxindex) {
-    nearest = maxindex;
-    return maxindex - order;
-  }
-  float xl = xs[lowlim], xu = xs[uplim];
-  if (asc) nearest = (x - xl) <= (xu - x) ? lowlim : uplim;
-  else nearest = (xl - x) <= (x - xu) ? lowlim : uplim;
+  if (lowlim < 0) return 0;
+  if (uplim > maxindex) return maxindex - order;
+  T xl = xs[lowlim], xu = xs[uplim];
   int first = lowlim, last = uplim;
   while ((last - first) < order && first > 0 && last < maxindex) {
-    if ((((xl - xs[first - 1]) + xu) - xs[last + 1] >= 0.f) == asc) last += 1;
+    if ((((xl - xs[first - 1]) + xu) - xs[last + 1] >= (T)0) == asc) last += 1;
     else first -= 1;
   }
   if (last >= maxindex) return maxindex - order;
@@ -338,18 +348,18 @@ MD int nu_locate(const float* xs, int n, int order, float x, int& nearest) {
  * interpolant sum_i w_i f_i is the polynomial Neville's scheme
  * (NULagrangeInterpolation.cu:85-112) evaluates.  Weights are formed once
  * per axis and shared by every row of a multi-dimensional window. */
-template <int ORDER>
-MD void nu_weights(const float* xs, float x, float* w) {
+template <typename T, int ORDER>
+MD void nu_weights(const T* xs, T x, T* w) {
   /* differences are taken in units of the window's span, so that products
-   * of ORDER of them stay far from FP32 underflow whatever the grid's units
+   * of ORDER of them stay far from underflow whatever the grid's units
    * (energies in J are ~1e-17) */
-  float xv[ORDER + 1];
+  T xv[ORDER + 1];
 #pragma unroll
   for (int i = 0; i <= ORDER; ++i) xv[i] = xs[i];
-  const float inv = 1.f / (xv[ORDER] - xv[0]);
+  const T inv = (T)1 / (xv[ORDER] - xv[0]);
 #pragma unroll
   for (int i = 0; i <= ORDER; ++i) {
-    float num = 1.f, den = 1.f;
+    T num = 1, den = 1;
 #pragma unroll
     for (int j = 0; j <= ORDER; ++j)
       if (j != i) {
@@ -360,125 +370,120 @@ MD void nu_weights(const float* xs, float x, float* w) {
   }
 }
 
-template <int ORDER>
-MD float nu_dot(const float* w, const float* f) {
-  float y = 0.f;
+template <typename T, int ORDER>
+MD T nu_dot(const T* w, const T* f) {
+  T y = 0;
 #pragma unroll
   for (int i = 0; i <= ORDER; ++i) y += w[i] * f[i];
   return y;
 }
 
-template <int ORDER>
-MD float nu_poly(const float* xs, const float* f, float x) {
-  float w[ORDER + 1];
-  nu_weights<ORDER>(xs, x, w);
-  return nu_dot<ORDER>(w, f);
+template <typename T, int ORDER>
+MD T nu_poly(const T* xs, const T* f, T x) {
+  T w[ORDER + 1];
+  nu_weights<T, ORDER>(xs, x, w);
+  return nu_dot<T, ORDER>(w, f);
 }
 
-template <int ORDER>
-MD float nu_d1(const float* f, const float* xs, int n, float x) {
-  int nearest;
-  int i0 = nu_locate(xs, n, ORDER, x, nearest);
-  return nu_poly<ORDER>(xs + i0, f + i0, x);
+/* NULagrangeInterpolation::d1 (:144-158). */
+template <typename T, int ORDER>
+MD T nu_d1(const DevScene& S, const DNUTable& t, T x0) {
+  const T* d = nu_base<T>(S);
+  const T* xa = d + t.xoff[0];
+  int i0 = nu_locate<T>(xa, t.n[0], ORDER, x0);
+  return nu_poly<T, ORDER>(xa + i0, d + t.voff + i0, x0);
 }
 
-/* NULagrangeInterpolation::d2 (:169-192); f is an [na][nb] block. */
-template <int ORDER>
-MD float nu_d2(const float* f, const float* xa, int na, const float* xb, int nb, float x0, float x1) {
-  int nearest;
-  int i0 = nu_locate(xa, na, ORDER, x0, nearest);
-  int j0 = nu_locate(xb, nb, ORDER, x1, nearest);
-  float wb[ORDER + 1], y[ORDER + 1];
-  nu_weights<ORDER>(xb + j0, x1, wb);
+/* NULagrangeInterpolation::d2 (:169-192); values are an [na][nb] block. */
+template <typename T, int ORDER>
+MD T nu_d2(const DevScene& S, const DNUTable& t, T x0, T x1) {
+  const T* d = nu_base<T>(S);
+  const T* xa = d + t.xoff[0];
+  const T* xb = d + t.xoff[1];
+  const T* f = d + t.voff;
+  const int nb = t.n[1];
+  int i0 = nu_locate<T>(xa, t.n[0], ORDER, x0);
+  int j0 = nu_locate<T>(xb, nb, ORDER, x1);
+  T wb[ORDER + 1], y[ORDER + 1];
+  nu_weights<T, ORDER>(xb + j0, x1, wb);
 #pragma unroll
-  for (int i = 0; i <= ORDER; ++i) y[i] = nu_dot<ORDER>(wb, f + (size_t)(i0 + i) * nb + j0);
-  return nu_poly<ORDER>(xa + i0, y, x0);
+  for (int i = 0; i <= ORDER; ++i) y[i] = nu_dot<T, ORDER>(wb, f + (size_t)(i0 + i) * nb + j0);
+  return nu_poly<T, ORDER>(xa + i0, y, x0);
 }
 
 /* NULagrangeInterpolation::d3 (:215-240); compat_d3 reproduces line 236
- * (outer abscissae taken from the start of the grid). */
-template <int ORDER>
-MD float nu_d3(const DevScene& S, const DNUTable& t, float x0, float x1, float x2, bool compat_d3) {
-  const float* xa = S.nu_data + t.xoff[0];
-  const float* xb = S.nu_data + t.xoff[1];
-  const float* xc = S.nu_data + t.xoff[2];
-  const float* v = S.nu_data + t.voff;
-  int nearest;
-  int i0 = nu_locate(xa, t.n[0], ORDER, x0, nearest);
-  int j0 = nu_locate(xb, t.n[1], ORDER, x1, nearest);
-  int k0 = nu_locate(xc, t.n[2], ORDER, x2, nearest);
-  float wb[ORDER + 1], wc[ORDER + 1], y[ORDER + 1];
-  nu_weights<ORDER>(xb + j0, x1, wb);
-  nu_weights<ORDER>(xc + k0, x2, wc);
+ * (outer abscissae taken from the start of the grid).  voff overrides the
+ * value offset (the 4-D lookup walks 3-D slices). */
+template <typename T, int ORDER>
+MD T nu_d3(const DevScene& S, const DNUTable& t, int ax, int voff, T x0, T x1, T x2, bool compat_d3) {
+  const T* d = nu_base<T>(S);
+  const T* xa = d + t.xoff[ax];
+  const T* xb = d + t.xoff[ax + 1];
+  const T* xc = d + t.xoff[ax + 2];
+  const T* v = d + voff;
+  const int nb = t.n[ax + 1], nc = t.n[ax + 2];
+  int i0 = nu_locate<T>(xa, t.n[ax], ORDER, x0);
+  int j0 = nu_locate<T>(xb, nb, ORDER, x1);
+  int k0 = nu_locate<T>(xc, nc, ORDER, x2);
+  T wb[ORDER + 1], wc[ORDER + 1], y[ORDER + 1];
+  nu_weights<T, ORDER>(xb + j0, x1, wb);
+  nu_weights<T, ORDER>(xc + k0, x2, wc);
 #pragma unroll 1
   for (int i = 0; i <= ORDER; ++i) {
-    float z[ORDER + 1];
-    const float* blk = v + (size_t)(i0 + i) * t.n[1] * t.n[2];
+    T z[ORDER + 1];
+    const T* blk = v + (size_t)(i0 + i) * nb * nc;
+#pragma unroll 1
+    for (int j = 0; j <= ORDER; ++j) {
+      const T zj = nu_dot<T, ORDER>(wc, blk + (size_t)(j0 + j) * nc + k0);
 #pragma unroll
-    for (int j = 0; j <= ORDER; ++j) z[j] = nu_dot<ORDER>(wc, blk + (size_t)(j0 + j) * t.n[2] + k0);
-    const float yi = nu_dot<ORDER>(wb, z);
+      for (int q = 0; q <= ORDER; ++q)
+        if (q == j) z[q] = zj;
+    }
+    const T yi = nu_dot<T, ORDER>(wb, z);
 #pragma unroll
     for (int q = 0; q <= ORDER; ++q)
       if (q == i) y[q] = yi;
   }
-  return nu_poly<ORDER>(compat_d3 ? xa : xa + i0, y, x0);
+  return nu_poly<T, ORDER>(compat_d3 ? xa : xa + i0, y, x0);
 }
 
-template <int ORDER>
-MD float nu_d4(const DevScene& S, const DNUTable& t, const float* x, bool compat_d3) {
-  const float* xa = S.nu_data + t.xoff[0];
-  int nearest;
-  int i0 = nu_locate(xa, t.n[0], ORDER, x[0], nearest);
-  float y[ORDER + 1];
+/* NULagrangeInterpolation::d4 (:259-306). */
+template <typename T, int ORDER>
+MD T nu_d4(const DevScene& S, const DNUTable& t, const T* x, bool compat_d3) {
+  const T* xa = nu_base<T>(S) + t.xoff[0];
+  int i0 = nu_locate<T>(xa, t.n[0], ORDER, x[0]);
+  T y[ORDER + 1];
 #pragma unroll
-  for (int i = 0; i <= ORDER; ++i) {
-    DNUTable sub;
-    sub.dim = 3;
-    for (int k = 0; k < 3; ++k) {
-      sub.n[k] = t.n[k + 1];
-      sub.xoff[k] = t.xoff[k + 1];
-    }
-    sub.voff = t.voff + (i0 + i) * t.n[1] * t.n[2] * t.n[3];
-    y[i] = nu_d3<ORDER>(S, sub, x[1], x[2], x[3], compat_d3);
-  }
-  return nu_poly<ORDER>(xa + i0, y, x[0]);
+  for (int i = 0; i <= ORDER; ++i)
+    y[i] = nu_d3<T, ORDER>(S, t, 1, t.voff + (i0 + i) * t.n[1] * t.n[2] * t.n[3], x[1], x[2], x[3], compat_d3);
+  return nu_poly<T, ORDER>(xa + i0, y, x[0]);
 }
 
-template <int ORDER>
-MD float nu_interp_o(const DevScene& S, int table, const float* x, bool compat_d3) {
+template <typename T, int ORDER>
+MD T nu_interp_o(const DevScene& S, int table, const T* x, bool compat_d3) {
   const DNUTable& t = S.nu[table];
-  const float* v = S.nu_data + t.voff;
   switch (t.dim) {
-    case 1: return nu_d1<ORDER>(v, S.nu_data + t.xoff[0], t.n[0], x[0]);
-    case 2: return nu_d2<ORDER>(v, S.nu_data + t.xoff[0], t.n[0], S.nu_data + t.xoff[1], t.n[1], x[0], x[1]);
-    case 3: return nu_d3<ORDER>(S, t, x[0], x[1], x[2], compat_d3);
-    case 4: return nu_d4<ORDER>(S, t, x, compat_d3);
+    case 1: return nu_d1<T, ORDER>(S, t, x[0]);
+    case 2: return nu_d2<T, ORDER>(S, t, x[0], x[1]);
+    case 3: return nu_d3<T, ORDER>(S, t, 0, t.voff, x[0], x[1], x[2], compat_d3);
+    case 4: return nu_d4<T, ORDER>(S, t, x, compat_d3);
   }
-  return 0.f;
+  return 0;
 }
 
-MD float nu_interp(const DevScene& S, int table, int order, const float* x, bool compat_d3) {
-  if (order == 1) return nu_interp_o<1>(S, table, x, compat_d3);
-  if (order == 2) return nu_interp_o<2>(S, table, x, compat_d3);
-  return nu_interp_o<3>(S, table, x, compat_d3);
-}
-
-/* Lookups in tables whose dimension the scene compiler has checked (the
- * inelastic mean free path is 1-D, the energy-loss and SE-energy tables
- * 2-D, the angle table 3-D). */
-template <int ORDER>
-MD float nu_table1(const DevScene& S, int table, float x0) {
-  const DNUTable& t = S.nu[table];
-  return nu_d1<ORDER>(S.nu_data + t.voff, S.nu_data + t.xoff[0], t.n[0], x0);
-}
-template <int ORDER>
-MD float nu_table2(const DevScene& S, int table, float x0, float x1) {
-  const DNUTable& t = S.nu[table];
-  return nu_d2<ORDER>(S.nu_data + t.voff, S.nu_data + t.xoff[0], t.n[0], S.nu_data + t.xoff[1], t.n[1], x0, x1);
-}
-template <int ORDER>
-MD float nu_table3(const DevScene& S, int table, float x0, float x1, float x2, bool compat_d3) {
-  return nu_d3<ORDER>(S, S.nu[table], x0, x1, x2, compat_d3);
+/* NUTableInterpolation::interpolate (NUTableInterpolation.cu:98-115) in
+ * the arithmetic the transport uses for that table's dimension: FP32 for
+ * 1-D (mean free path), FP64 otherwise. */
+MD double nu_interp(const DevScene& S, int table, int order, const double* x, bool compat_d3) {
+  if (S.nu[table].dim == 1) {
+    const float xf = (float)x[0];
+    if (order == 1) return nu_interp_o<float, 1>(S, table, &xf, compat_d3);
+    if (order == 2) return nu_interp_o<float, 2>(S, table, &xf, compat_d3);
+    return nu_interp_o<float, 3>(S, table, &xf, compat_d3);
+  }
+  if (order == 1) return nu_interp_o<double, 1>(S, table, x, compat_d3);
+  if (order == 2) return nu_interp_o<double, 2>(S, table, x, compat_d3);
+  return nu_interp_o<double, 3>(S, table, x, compat_d3);
 }
 
 /* ------------------------------------------------------ mechanisms ----- */
@@ -516,7 +521,7 @@ MD float mech_rate(const DevScene& S, const DMech& m, float EeV) {
     case MONSEL_MECH_TABULATED_INELASTIC: { /* TabulatedInelasticSM.cu:436-459 */
       float kE = (EeV + m.p[TP_OFFSET]) * kEcharge;
       if (kE < m.p[TP_IIMFPDOM0]) return 0.f;
-      return m.p[TP_RATEMULT] * nu_table1<1>(S, m.table[0], kE) * 1.0e-9f;
+      return m.p[TP_RATEMULT] * nu_d1<float, 1>(S, S.nu[m.table[0]], kE) * 1.0e-9f;
     }
   }
   return 0.f;
@@ -565,13 +570,13 @@ MD float e0_from_dispersion(float Eq, float deltaE) {
 }
 
 /* TabulatedInelasticSM::pickBE (:153-207), default branching ratios; J. */
-MD float pick_be(const DMech& m, float Eq, float deltaE) {
-  int ncore = (int)m.p[TP_NCORE];
-  if (ncore > 0 && deltaE <= m.p[TP_CORE0]) return 0.f;
-  float eIon = m.p[TP_E0DISP] != 0.f ? e0_from_dispersion(Eq, deltaE) : deltaE;
+MD double pick_be(const DMech& m, double Eq, double deltaE) {
+  const int ncore = (int)m.q[TP_NCORE];
+  if (ncore > 0 && deltaE <= m.q[TP_CORE0]) return 0.;
+  const double eIon = m.q[TP_E0DISP] != 0. ? (double)e0_from_dispersion((float)Eq, (float)deltaE) : deltaE;
   int i = 0;
-  while (i < ncore && m.p[TP_CORE0 + i] <= eIon) ++i;
-  return i == 0 ? 0.f : m.p[TP_CORE0 + i - 1];
+  while (i < ncore && m.q[TP_CORE0 + i] <= eIon) ++i;
+  return i == 0 ? 0. : m.q[TP_CORE0 + i - 1];
 }
 
 MD void isotropic_dir(float u1, float u2, float& dx, float& dy, float& dz) {
@@ -585,40 +590,41 @@ MD void isotropic_dir(float u1, float u2, float& dx, float& dy, float& dz) {
   dz = ct;
 }
 
-/* TabulatedInelasticSM::scatter (TabulatedInelasticSM.cu:209-434).  Table
- * lookups in SI units (J), results converted back to eV. */
+/* TabulatedInelasticSM::scatter (TabulatedInelasticSM.cu:209-434) in FP64,
+ * in the reference's SI units (J); only the final rotations are FP32.  The
+ * parameters come from DMech::q (J). */
 MD Spawn tabulated_scatter(const DevScene& S, const DMech& m, Elec& e, const float* u) {
   Spawn se;
   se.has = false;
   const bool cd3 = (S.compat & MONSEL_COMPAT_D3_ABSCISSAE) != 0;
-  const float J = kEcharge;
-  float kE0 = e.E * J;
-  float kE = kE0 + m.p[TP_OFFSET] * J;
-  if (kE < m.p[TP_EIDOM0]) return se;
-  const float rnd0 = u[2], rnd1 = u[3], rnd2 = u[4], rnd3 = u[5];
-  float deltaE = kE * nu_table2<3>(S, m.table[1], kE, rnd0);
-  float gap = m.p[TP_GAP] * J, bandgap = m.p[TP_BANDGAP] * J;
-  if (deltaE < gap && deltaE > 0.95f * gap) deltaE = gap;
-  float d0x = e.dx, d0y = e.dy, d0z = e.dz;
-  float theta = 0.f, sphi = 0.f, cphi = 1.f, phi = 0.f;
-  float offsetFermi = m.p[TP_OFFSETFERMI] * J;
+  const double J = 1.60217653e-19;
+  const double kE0 = (double)e.E * J;
+  const double kE = kE0 + m.q[TP_OFFSET];
+  if (kE < m.q[TP_EIDOM0]) return se;
+  const double rnd0 = u[2], rnd1 = u[3], rnd2 = u[4], rnd3 = u[5];
+  double deltaE = kE * nu_d2<double, 3>(S, S.nu[m.table[1]], kE, rnd0);
+  const double gap = m.q[TP_GAP], bandgap = m.q[TP_BANDGAP];
+  if (deltaE < gap && deltaE > 0.95 * gap) deltaE = gap;
+  const float d0x = e.dx, d0y = e.dy, d0z = e.dz;
+  double theta = 0.;
+  float sphi = 0.f, cphi = 1.f;
+  const double offsetFermi = m.q[TP_OFFSETFERMI];
   if (deltaE >= bandgap) {
-    float red = (deltaE - gap) / (kE - offsetFermi - 2.f * gap);
-    red = red > 1.f ? 1.f : (red < 0.f ? 0.f : red);
-    theta = nu_table3<3>(S, m.table[2], kE, red, rnd1, cd3);
-    phi = kTwoPI * rnd2;
-    __sincosf(phi, &sphi, &cphi);
+    double red = (deltaE - gap) / (kE - offsetFermi - 2. * gap);
+    red = red > 1. ? 1. : (red < 0. ? 0. : red);
+    theta = nu_d3<double, 3>(S, S.nu[m.table[2]], 0, S.nu[m.table[2]].voff, kE, red, rnd1, cd3);
+    __sincosf(kTwoPI * (float)rnd2, &sphi, &cphi);
     float st, ct;
-    __sincosf(theta, &st, &ct);
+    sincosf((float)theta, &st, &ct);
     rotate_dir(e.dx, e.dy, e.dz, ct, st, cphi, sphi);
   }
-  e.E = (kE0 - deltaE) / J;
+  e.E = (float)((kE0 - deltaE) / J);
   if (deltaE < bandgap) return se;
-  float Eq = (2.f * kE) - deltaE - (2.f * sqrtf(kE * (kE - deltaE)) * cosf(theta));
-  float cb = m.p[TP_CB] * J, minEgen = m.p[TP_MINEGEN] * J, bEref = m.p[TP_BEREF] * J;
-  float offset = m.p[TP_OFFSET] * J;
-  float energySE;
-  int method = m.method;
+  const double Eq = (2. * kE) - deltaE - (2. * sqrt(kE * (kE - deltaE)) * (double)cosf((float)theta));
+  const double cb = m.q[TP_CB], minEgen = m.q[TP_MINEGEN], bEref = m.q[TP_BEREF];
+  const double offset = m.q[TP_OFFSET];
+  double energySE;
+  const int method = m.method;
   /* SE direction relative to the PE's original direction: thetaSE = pi/2 - theta,
    * phiSE = phi + pi */
   auto rel_dir = [&](float& sx, float& sy, float& sz) {
@@ -626,13 +632,13 @@ MD Spawn tabulated_scatter(const DevScene& S, const DMech& m, Elec& e, const flo
     sy = d0y;
     sz = d0z;
     float st, ct;
-    __sincosf(0.5f * kPI - theta, &st, &ct);
+    sincosf((float)(0.5 * 3.14159265358979323846 - theta), &st, &ct);
     rotate_dir(sx, sy, sz, ct, st, -cphi, -sphi);
   };
-  auto se0 = [&](float dE) {
-    float e0 = nu_table2<3>(S, m.table[3], dE, rnd3);
-    if (e0 < m.p[TP_SE0RANGE0] * J) e0 = m.p[TP_SE0RANGE0] * J;
-    else if (e0 > m.p[TP_SE0RANGE1] * J) e0 = m.p[TP_SE0RANGE1] * J;
+  auto se0 = [&](double dE) {
+    double e0 = nu_d2<double, 3>(S, S.nu[m.table[3]], dE, rnd3);
+    if (e0 < m.q[TP_SE0RANGE0]) e0 = m.q[TP_SE0RANGE0];
+    else if (e0 > m.q[TP_SE0RANGE1]) e0 = m.q[TP_SE0RANGE1];
     return e0;
   };
   if (method == 1) {
@@ -640,44 +646,44 @@ MD Spawn tabulated_scatter(const DevScene& S, const DMech& m, Elec& e, const flo
     if (energySE + cb < minEgen) return se;
     rel_dir(se.dx, se.dy, se.dz);
   } else if (method == 2) {
-    float be = pick_be(m, Eq, deltaE);
-    if (be > 0.f) energySE = (deltaE + bEref) - be;
+    const double be = pick_be(m, Eq, deltaE);
+    if (be > 0.) energySE = (deltaE + bEref) - be;
     else energySE = (deltaE + se0(deltaE)) - offset;
     if (energySE + cb < minEgen) return se;
     rel_dir(se.dx, se.dy, se.dz);
   } else {
-    float be = pick_be(m, Eq, deltaE);
-    if (be > 0.f) {
+    const double be = pick_be(m, Eq, deltaE);
+    if (be > 0.) {
       energySE = (deltaE + bEref) - be;
       if (energySE + cb < minEgen) return se;
       isotropic_dir(u[6], u[7], se.dx, se.dy, se.dz);
     } else {
-      float root = 2.f * sqrtf(offsetFermi * (offsetFermi + deltaE));
-      float sum = (2.f * offsetFermi) + deltaE;
+      const double root = 2. * sqrt(offsetFermi * (offsetFermi + deltaE));
+      const double sum = (2. * offsetFermi) + deltaE;
       if ((sum - root) <= Eq && Eq <= (sum + root)) {
         /* simESEf (TabulatedInelasticSM.cu:87-101) */
-        float q = sqrtf(Eq);
-        float kz = (deltaE - Eq) / 2.f / q;
-        float kzf = kz + q;
-        float Ezq = kzf * kzf;
-        float minE = (offsetFermi + bandgap) - Ezq;
-        if (minE < 0.f) minE = 0.f;
-        float maxE = offsetFermi - (kz * kz);
-        float Exy = (minE * (1.f - rnd3)) + (maxE * rnd3);
-        float ESEf = Exy + Ezq;
-        float thSE = acosf(kzf / sqrtf(ESEf));
+        const double q = sqrt(Eq);
+        const double kz = (deltaE - Eq) / 2. / q;
+        const double kzf = kz + q;
+        const double Ezq = kzf * kzf;
+        double minE = (offsetFermi + bandgap) - Ezq;
+        if (minE < 0.) minE = 0.;
+        const double maxE = offsetFermi - (kz * kz);
+        const double Exy = (minE * (1. - rnd3)) + (maxE * rnd3);
+        const double ESEf = Exy + Ezq;
+        const float thSE = acosf((float)(kzf / sqrt(ESEf)));
         energySE = ESEf - offset;
         if (energySE + cb < minEgen) return se;
         /* TabulatedInelasticSM.cu:388-395: the q-vector direction (pi/2 - theta,
          * phi + pi) is deflected by (thSE, 2 pi U) in the PE frame, and the SE
          * is the PE's original direction rotated by the resulting angles. */
         float st, ct;
-        __sincosf(0.5f * kPI - theta, &st, &ct);
+        sincosf((float)(0.5 * 3.14159265358979323846 - theta), &st, &ct);
         float wx, wy, wz;
         dir_from_angles(ct, st, -cphi, -sphi, wx, wy, wz);
         float sp2, cp2, st2, ct2;
         __sincosf(kTwoPI * u[6], &sp2, &cp2);
-        __sincosf(thSE, &st2, &ct2);
+        sincosf(thSE, &st2, &ct2);
         rotate_dir(wx, wy, wz, ct2, st2, cp2, sp2);
         float wst = __builtin_sqrtf(fmaxf(0.f, wx * wx + wy * wy));
         float wcp = 1.f, wsp = 0.f;
@@ -697,7 +703,7 @@ MD Spawn tabulated_scatter(const DevScene& S, const DMech& m, Elec& e, const flo
     }
   }
   se.has = true;
-  se.E = energySE / J;
+  se.E = (float)(energySE / J);
   return se;
 }
 

@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define MONSEL_ABI_VERSION 1u
+#define MONSEL_ABI_VERSION 2u
 
 typedef enum monsel_status {
   MONSEL_OK = 0,
@@ -269,8 +269,11 @@ typedef struct monsel_job_stats {
 typedef struct monsel_tuning {
   int32_t block_size;        /* threads per workgroup (multiple of 64, <= 256) */
   int32_t blocks_per_cu;     /* persistent workgroups per CU */
-  int32_t max_generations;   /* secondary generations per job */
-  int32_t queue_capacity;    /* electrons per generation buffer (0 = sized from the job) */
+  int32_t max_generations;   /* transport launches per job before its electrons are dropped */
+  int32_t queue_capacity;    /* electrons per generation buffer (0 = from the device memory) */
+  int32_t max_jobs_in_flight;   /* jobs sharing the transport launches (0 = 1024) */
+  int32_t primaries_per_launch; /* new primaries admitted per launch (0 = queue capacity) */
+  int32_t steps_per_launch;     /* regroup interval: steps an electron takes per launch (0 = unlimited) */
 } monsel_tuning;
 
 typedef struct monsel_ctx_s* monsel_ctx;

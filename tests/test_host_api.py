@@ -69,7 +69,8 @@ def test_library_exports_header(M):
     assert len(names) >= 20
     for n in names:
         assert hasattr(lib, n), n
-    assert lib.monsel_abi_version() == 1
+    version = int(re.search(r"#define MONSEL_ABI_VERSION (\d+)u", open(os.path.join(ROOT, "include", "monsel_gpu.h")).read()).group(1))
+    assert lib.monsel_abi_version() == version
 
 
 def test_scene_footprint_without_device(M):
