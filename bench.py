@@ -79,6 +79,47 @@ def cpu_baseline(sc, centers, target_s=10.0):
                       f"in {dt:.1f} s on the FP64 oracle"}
 
 
+def batched_leg(n_jobs, tuning):
+    """The batch configuration: n_jobs single-pixel yield jobs (E0 log-uniform
+    0.2-5 keV, Si/PMMA/Cu, flat or 20 nm step edge, 1e2-1e5 primaries)
+    submitted together on an (auto)tuned context, against the same jobs one
+    at a time on a context that admits one job at a time."""
+    import numpy as np
+    from cudamonsel_amd import serving
+    from cudamonsel_amd.abi import Engine
+    jobs = serving.batch_workload(n_jobs, seed=4)
+    prim = sum(j.primaries for j in jobs)
+    with Engine(**tuning) as eng:
+        serving.run_batch(eng, jobs[:8])   # warm-up: scenes, code paths
+        res, t_b = serving.run_batch(eng, jobs)
+    with Engine(max_jobs_in_flight=1) as eng:
+        serving.run_batch(eng, jobs[:8], plain=True)
+        res_p, t_p = serving.run_batch(eng, jobs, plain=True)
+    same = all(np.array_equal(a["back"], b["back"]) and np.array_equal(a["se"], b["se"]) for a, b in zip(res, res_p))
+    return {"jobs": n_jobs, "primaries": prim, "batched_primaries_per_s": round(prim / t_b, 1),
+            "plain_primaries_per_s": round(prim / t_p, 1), "speedup": round(t_p / t_b, 2),
+            "batched_s": round(t_b, 3), "plain_s": round(t_p, 3), "tuning": tuning,
+            "results_identical": bool(same)}
+
+
+def served_leg(duration_s, rate):
+    """The serving configuration on a shortened stream: Poisson arrivals at
+    `rate`/s for duration_s s of SEM-image requests (side 32-256 px, 50-500
+    primaries/pixel, 0.3-2 keV, random PMMA line arrays or contact holes on
+    Si), continuous batching against one request at a time."""
+    from cudamonsel_amd import serving
+    from cudamonsel_amd.abi import Engine
+    reqs = serving.request_stream(rate_per_s=rate, duration_s=duration_s, seed=5)
+    out = {"stream": f"Poisson {rate}/s for {duration_s:g} s, {len(reqs)} requests, "
+                     f"{sum(r.primaries for r in reqs)} primaries"}
+    for name, plain in (("served", False), ("plain", True)):
+        with Engine() as eng:
+            r = serving.serve(eng, reqs, plain=plain)
+        out[name] = {k: round(v, 3) if isinstance(v, float) else v for k, v in r.items() if k != "latencies"}
+    out["images_per_hour_ratio"] = round(out["served"]["images_per_hour"] / out["plain"]["images_per_hour"], 3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +129,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="run one device copy of this size first (HBM-counter calibration under rocprof)")
+    ap.add_argument("--batch-jobs", type=int, default=512, help="jobs of the batched leg (0: skip)")
+    ap.add_argument("--serve-seconds", type=float, default=10.0, help="arrival window of the served leg (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=2.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: no multi-GPU path is built (a scan would shard by pixel "
@@ -157,6 +201,13 @@ def main():
             "bytes_per_step": BYTES_PER_STEP,
         },
     }
+    if args.batch_jobs > 0:
+        import json as _json
+        tfile = os.path.join(ROOT, "cudamonsel_amd", "data", "tuning_batch.json")
+        tuning = _json.load(open(tfile))["tuning"] if os.path.exists(tfile) else {}
+        out["batched"] = batched_leg(args.batch_jobs, tuning)
+    if args.serve_seconds > 0:
+        out["served"] = served_leg(args.serve_seconds, args.serve_rate)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc, centers)
     print(json.dumps(out), flush=True)

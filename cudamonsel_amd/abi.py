@@ -195,6 +195,20 @@ class Engine:
         self._keep[jid.value] = (c.shape[0], int(nbins))
         return jid.value
 
+    def poll(self, job_id):
+        done = ctypes.c_int(0)
+        _check(self.L.monsel_job_poll(self.ctx, ctypes.c_uint64(job_id), ctypes.byref(done)), "monsel_job_poll")
+        return bool(done.value)
+
+    def footprint(self, n_pixels, per_pixel, beam_eV, nbins=50):
+        """Device bytes a job of this shape holds while in flight."""
+        c = np.zeros((1, 3))
+        jd = JobDesc(beam_eV * 1.60217653e-19, 0.5e-9, int(n_pixels), int(per_pixel), c.ctypes.data_as(c_double_p),
+                     0, int(nbins), 0, 50.0, 0)
+        nb = ctypes.c_size_t(0)
+        _check(self.L.monsel_job_footprint(self.ctx, ctypes.byref(jd), ctypes.byref(nb)), "monsel_job_footprint")
+        return nb.value
+
     def result(self, job_id, timeout_s=None):
         """Waits for the job, returns its tallies and statistics, releases it."""
         us = -1 if timeout_s is None else int(timeout_s * 1e6)

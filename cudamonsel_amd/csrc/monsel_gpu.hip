@@ -104,7 +104,10 @@ struct JobRec {
   unsigned long long* prim_energy; /* [primaries]: summed back-escape energy, units of kEnergyQuantum eV */
   unsigned long long* hist; /* back energy [nbins+2], fwd energy [nbins+2], elevation [182], azimuth [362] */
   unsigned long long* stats; /* [8] */
+  float* wd_log;           /* optional: state of the first kWatchdogLog watchdog drops */
+  unsigned int* wd_count;
 };
+constexpr int kWatchdogLog = 16, kWatchdogFields = 10;
 
 /* One transport launch ("generation"): new primaries described by
  * segments, then the queued electrons of the previous launch. */
@@ -297,7 +300,7 @@ __device__ __forceinline__ int take_step(const DevScene& S, const JobRec* jobs, 
   }
   const DMSM& msm = S.msms[S.regions[region].msm];
   float u[kStepUniforms];
-  step_uniforms(e, kPhiloxKey0, kPhiloxKey1, u);
+  step_uniforms_lo(e, kPhiloxKey0, kPhiloxKey1, u);
   /* MONSEL_MaterialScatterModel::randomMeanPathLength (:64-91); each
    * mechanism's rate is kept in a per-lane LDS slot for the scatter choice */
   __shared__ float l_rate[kMaxMech][kLaneBlock];
@@ -326,6 +329,17 @@ __device__ __forceinline__ int take_step(const DevScene& S, const JobRec* jobs, 
   e.px = px + len * e.dx;
   e.py = py + len * e.dy;
   e.pz = pz + len * e.dz;
+  if (se.boundary && se.next == region) {
+    /* a grazing crossing: the point a nudge beyond the boundary is back in
+     * this region (a corner clipped within the nudge, or FP32 rounding
+     * between the crossing walk and the containment test).  The step still
+     * ends there as in RegionBase::findEndOfStep, but the electron is carried
+     * past the feature so that the next step cannot stop at it again. */
+    const float dd = nudge_nm(e.px, e.py, e.pz);
+    e.px += dd * e.dx;
+    e.py += dd * e.dy;
+    e.pz += dd * e.dz;
+  }
   float prevE = e.E;
   if (!msm.is_null) e.E += csd_compute(msm.csd, len, e.E);
   e.steps++;
@@ -346,6 +360,7 @@ __device__ __forceinline__ int take_step(const DevScene& S, const JobRec* jobs, 
       cum += rate;
     }
     bool trapped = false;
+    step_uniforms_hi(e, kPhiloxKey0, kPhiloxKey1, u);
     sp = mech_scatter(S, S.mechs[msm.mech0 + idx], e, prevE, rate, trapped, u);
     if (sp.has) {
       /* trackSecondaryElectron (MonteCarloSS.cu:165-174) */
@@ -598,7 +613,19 @@ k_transport(DevScene S, GenDev G) {
     if (alive && (res != STEP_CONTINUE || requeue)) {
       alive = false;
       if (res == STEP_ESCAPED) nesc++;
-      if (res == STEP_WATCHDOG) nwd++;
+      if (res == STEP_WATCHDOG) {
+        nwd++;
+        const JobRec& J = G.jobs[e.job];
+        if (J.wd_log) {
+          const unsigned int k = atomicAdd(J.wd_count, 1u);
+          if (k < (unsigned int)kWatchdogLog) {
+            float* w = J.wd_log + (size_t)k * kWatchdogFields;
+            w[0] = e.px; w[1] = e.py; w[2] = e.pz;
+            w[3] = e.dx; w[4] = e.dy; w[5] = e.dz;
+            w[6] = e.E; w[7] = (float)e.region; w[8] = (float)e.steps; w[9] = (float)e.primary;
+          }
+        }
+      }
     }
   }
   /* flush counters: reduced across the wave when every lane's counters
@@ -1438,7 +1465,7 @@ struct Job {
   uint64_t reserved = 0;      /* queue entries reserved for its electrons in flight */
   uint32_t outstanding = 0;   /* its electrons in the current queue */
   int generations = 0;
-  DBuf prim_counts, prim_fwd, prim_energy, px, hist, stats_dev, centers_dev;
+  DBuf prim_counts, prim_fwd, prim_energy, px, hist, stats_dev, centers_dev, wd;
 };
 
 }  // namespace
@@ -1588,6 +1615,14 @@ void setup_job(monsel_ctx_s* C, Job& job) {
   R.prim_energy = job.prim_energy.as<unsigned long long>();
   R.hist = job.hist.as<unsigned long long>();
   R.stats = job.stats_dev.as<unsigned long long>();
+  /* MONSEL_DEBUG_WATCHDOG=1: keep the state of the first watchdog drops and
+   * print it when the job completes */
+  if (getenv("MONSEL_DEBUG_WATCHDOG")) {
+    job.wd.ensure(kWatchdogLog * kWatchdogFields * 4 + 16);
+    HIP_TRY(hipMemsetAsync(job.wd.p, 0, kWatchdogLog * kWatchdogFields * 4 + 16, C->stream));
+    R.wd_log = job.wd.as<float>();
+    R.wd_count = (unsigned int*)(job.wd.as<float>() + kWatchdogLog * kWatchdogFields);
+  }
   HIP_TRY(hipMemcpyAsync(C->jobtab.as<JobRec>() + job.slot, &R, sizeof(JobRec), hipMemcpyHostToDevice, C->stream));
   HIP_TRY(hipStreamSynchronize(C->stream));
 }
@@ -1635,6 +1670,18 @@ void complete_job(monsel_ctx_s* C, Job& job) {
   job.stats.generations = job.generations;
   job.stats.device_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - job.t_start).count();
+  if (job.wd.p) {
+    std::vector<float> w(kWatchdogLog * kWatchdogFields + 4);
+    HIP_TRY(hipMemcpy(w.data(), job.wd.p, w.size() * 4, hipMemcpyDeviceToHost));
+    unsigned int n = 0;
+    memcpy(&n, w.data() + kWatchdogLog * kWatchdogFields, 4);
+    fprintf(stderr, "[monsel] job %llu: %u watchdog drops\n", (unsigned long long)job.id, n);
+    for (unsigned int k = 0; k < std::min<unsigned int>(n, kWatchdogLog); ++k) {
+      const float* v = w.data() + k * kWatchdogFields;
+      fprintf(stderr, "[monsel]   p=(%.6g %.6g %.6g) nm d=(%.6g %.6g %.6g) E=%.6g eV region=%d steps=%.0f primary=%.0f\n",
+              v[0], v[1], v[2], v[3], v[4], v[5], v[6], (int)v[7], v[8], v[9]);
+    }
+  }
 }
 
 void release_job(monsel_ctx_s* C, Job& job) {
@@ -1647,6 +1694,7 @@ void release_job(monsel_ctx_s* C, Job& job) {
   job.hist.reset();
   job.stats_dev.reset();
   job.centers_dev.reset();
+  job.wd.reset();
   job.sd.reset();
 }
 

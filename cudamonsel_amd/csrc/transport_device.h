@@ -97,14 +97,21 @@ MD float u01(uint32_t x) { return (float)(x >> 9) * 1.1920928955078125e-07f + 5.
  * in the order the reference's scatter() draws them (at most six). */
 constexpr int kStepUniforms = 8;
 
-MD void step_uniforms(const Elec& e, uint32_t k0, uint32_t k1, float* u) {
-  uint32_t a0, a1, a2, a3, b0, b1, b2, b3;
+/* u[0..3] from block 2s (drawn when the step starts) */
+MD void step_uniforms_lo(const Elec& e, uint32_t k0, uint32_t k1, float* u) {
+  uint32_t a0, a1, a2, a3;
   philox4x32_10(2u * e.steps, e.lin_lo, e.lin_hi, e.primary, k0, k1, a0, a1, a2, a3);
-  philox4x32_10(2u * e.steps + 1u, e.lin_lo, e.lin_hi, e.primary, k0, k1, b0, b1, b2, b3);
   u[0] = u01(a0);
   u[1] = u01(a1);
   u[2] = u01(a2);
   u[3] = u01(a3);
+}
+
+/* u[4..7] from block 2s+1, drawn only by a scatter event (s = the step
+ * count before the step, `steps` has been advanced since) */
+MD void step_uniforms_hi(const Elec& e, uint32_t k0, uint32_t k1, float* u) {
+  uint32_t b0, b1, b2, b3;
+  philox4x32_10(2u * (e.steps - 1u) + 1u, e.lin_lo, e.lin_hi, e.primary, k0, k1, b0, b1, b2, b3);
   u[4] = u01(b0);
   u[5] = u01(b1);
   u[6] = u01(b2);

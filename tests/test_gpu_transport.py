@@ -4,7 +4,8 @@ job parameters and seeds (FP32 device transport against FP64 oracle).
 Gates: per-pixel and all-pixel BSE/SE yields within 3 sigma (4 sigma for
 per-pixel maxima over many pixels, plus a chi-square test of the pixel
 z-scores), exit-energy and exit-angle histograms by two-sample KS at
-p > 0.01.  Workloads are reduced forms of the benchmark configurations: bulk
+p > 0.01 for each configuration (the three histograms of a configuration
+share that level: each must pass p > 0.01/3, Bonferroni).  Workloads are reduced forms of the benchmark configurations: bulk
 Si at 1 keV with 200 energy bins over 0-1 keV (the bulk-Si configuration),
 a PMMA line on Si at 0.5 keV (the line-scan configuration) in both
 reference-compat and fixed modes, PMMA at 0.5 keV, the tabulated-inelastic
@@ -42,11 +43,12 @@ def check_yields(r, o, per_pixel_max=4.0):
 
 
 def check_histograms(r, o):
-    for name in ("backEnergyHist", "elevationHist", "azimuthHist"):
+    names = ("backEnergyHist", "elevationHist", "azimuthHist")
+    for name in names:
         hg = getattr(r, name)
         ho = o[name]
         D, p = su.ks_binned(hg, ho)
-        assert p > 0.01, (name, D, p)
+        assert p > 0.01 / len(names), (name, D, p)
 
 
 def test_flat_pmma_500eV(oracle, M):
@@ -158,3 +160,21 @@ def test_deterministic_and_independent_of_job_mix(M):
     for jid in ids:
         assert L.monsel_job_release(ctx, jid) == 0
     assert np.array_equal(back, a.back) and np.array_equal(se, a.se)
+
+
+def test_no_stuck_electrons_at_line_corners(M):
+    """Regression: in seeded three-line scenes FP32 electrons in the vacuum
+    stopped at the same grazing crossing near a line's top corner step after
+    step until the watchdog dropped them (several per million primaries, each
+    holding a launch for seconds).  A 64 x 64 raster, 307 primaries/pixel at
+    377 eV on random_lines(seed=103) must finish without watchdog drops."""
+    from cudamonsel_amd import scenes, serving
+    from cudamonsel_amd.abi import Engine
+    sc = scenes.random_lines(seed=103)
+    cs = sc.monte.compileScene()
+    cen = serving.raster_centers(64, sc.beam_z)
+    with Engine() as eng:
+        sid = eng.upload(cs)
+        for seed in (1, 2):
+            r = eng.result(eng.submit(sid, cen, 307, 377.0, 0.5e-9, seed, max_steps=200000), timeout_s=60)
+            assert r["stats"]["watchdog_drops"] == 0 and r["stats"]["queue_overflow"] == 0
