@@ -42,6 +42,11 @@ def build_scene():
     return sc, centers
 
 
+def log(msg):
+    """Progress on stderr (stdout carries only the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def copy_calibration(nbytes):
     """One device-to-device copy of nbytes through the HIP runtime the
     engine uses, so that a counter session sees a kernel of known traffic."""
@@ -89,12 +94,15 @@ def batched_leg(n_jobs, tuning):
     from cudamonsel_amd.abi import Engine
     jobs = serving.batch_workload(n_jobs, seed=4)
     prim = sum(j.primaries for j in jobs)
+    log(f"batched leg: {n_jobs} jobs, {prim} primaries, tuning {tuning}")
     with Engine(**tuning) as eng:
         serving.run_batch(eng, jobs[:8])   # warm-up: scenes, code paths
         res, t_b = serving.run_batch(eng, jobs)
+    log(f"batched: {t_b:.3f} s")
     with Engine(max_jobs_in_flight=1) as eng:
         serving.run_batch(eng, jobs[:8], plain=True)
         res_p, t_p = serving.run_batch(eng, jobs, plain=True)
+    log(f"plain: {t_p:.3f} s")
     same = all(np.array_equal(a["back"], b["back"]) and np.array_equal(a["se"], b["se"]) for a, b in zip(res, res_p))
     return {"jobs": n_jobs, "primaries": prim, "batched_primaries_per_s": round(prim / t_b, 1),
             "plain_primaries_per_s": round(prim / t_p, 1), "speedup": round(t_p / t_b, 2),
@@ -112,9 +120,12 @@ def served_leg(duration_s, rate):
     reqs = serving.request_stream(rate_per_s=rate, duration_s=duration_s, seed=5)
     out = {"stream": f"Poisson {rate}/s for {duration_s:g} s, {len(reqs)} requests, "
                      f"{sum(r.primaries for r in reqs)} primaries"}
+    log(f"served leg: {out['stream']}")
     for name, plain in (("served", False), ("plain", True)):
-        with Engine() as eng:
-            r = serving.serve(eng, reqs, plain=plain)
+        # served: the request with the fewest primaries left gets new launches first
+        with Engine(admission_order=0 if plain else 1) as eng:
+            r = serving.serve(eng, reqs, plain=plain, log=log)
+        log(f"{name}: {r['wall_s']:.2f} s, p50 {r['p50_s']:.2f} s, p99 {r['p99_s']:.2f} s")
         out[name] = {k: round(v, 3) if isinstance(v, float) else v for k, v in r.items() if k != "latencies"}
     out["images_per_hour_ratio"] = round(out["served"]["images_per_hour"] / out["plain"]["images_per_hour"], 3)
     return out
@@ -146,6 +157,7 @@ def main():
 
     sc, centers = build_scene()
     monte = sc.monte
+    log("line scan")
     n_prim = centers.shape[0] * args.per_pixel
     for _ in range(args.warmup):
         monte.runPixels(centers, args.per_pixel, 50, 50.0)

@@ -941,6 +941,8 @@ struct HostScene {
   std::vector<float> nudata;
   std::vector<double> nudata64;
   uint32_t compat = 0;
+  /* bound on tracked secondaries per eV of primary energy (queue reservation) */
+  double secondaries_per_eV = 0.;
   float chamber_r = 1e8f;
   float chamber_c[3] = {0, 0, 0};
 };
@@ -1132,6 +1134,10 @@ HostScene compile_scene(const monsel_scene_desc* d) {
           break;
         }
         case MONSEL_MECH_FITTED_INELASTIC: {
+          /* FittedInelSM.cu:20-25: one SE per E_SEgen of energy lost, and an
+           * SE (born at E_SEgen + E_F, the rate threshold) emits none; 1.5x
+           * covers the Poisson spread of a cascade */
+          H.secondaries_per_eV = std::max(H.secondaries_per_eV, 1.5 * kE / e.p[0]);
           double seE = e.p[0] + eFermi;
           dm.p[0] = (float)(seE / kE);
           dm.p[1] = (float)(seE / kE);
@@ -1154,6 +1160,8 @@ HostScene compile_scene(const monsel_scene_desc* d) {
           break;
         }
         case MONSEL_MECH_TABULATED_INELASTIC: {
+          /* SEs may cascade: one per 8 eV of primary energy at most */
+          H.secondaries_per_eV = std::max(H.secondaries_per_eV, 1.0 / 8.0);
           int method = (e.method != 2 && e.method != 3) ? 1 : e.method;
           dm.method = method;
           /* table dimensions and the cubic (order 3) windows the lookups use */
@@ -1562,7 +1570,9 @@ void setup_job(monsel_ctx_s* C, Job& job) {
   const int nbins = jd.n_energy_bins > 0 ? jd.n_energy_bins : 1;
   const double E0eV = jd.beam_energy / kE;
   job.total = (uint64_t)jd.n_pixels * (uint64_t)jd.primaries_per_pixel;
-  job.K = secondaries_per_primary(E0eV);
+  /* queue entries reserved per primary: itself plus its secondaries, bounded
+   * by the scene's inelastic mechanisms */
+  job.K = (uint32_t)std::min<double>(4096., 16. + E0eV * job.sd->host.secondaries_per_eV);
   job.launched = job.reserved = 0;
   job.outstanding = 0;
   job.generations = 0;
@@ -1708,10 +1718,16 @@ void run_generation(monsel_ctx_s* C) {
   std::vector<Job*> inflight;
   uint64_t reservedTotal = 0;
   for (auto& j : C->active) reservedTotal += j->reserved;
-  /* new primaries: FIFO over the active jobs, within the queue reservation */
+  /* new primaries within the queue reservation: oldest job first, or (with
+   * admission_order 1) the job with the fewest primaries left first */
+  std::vector<Job*> order;
+  for (auto& jp : C->active) order.push_back(jp.get());
+  if (C->tuning.admission_order == 1)
+    std::stable_sort(order.begin(), order.end(),
+                     [](const Job* a, const Job* b) { return a->total - a->launched < b->total - b->launched; });
   int nseg = 0;
   uint64_t nprim = 0;
-  for (auto& jp : C->active) {
+  for (Job* jp : order) {
     Job& j = *jp;
     if (j.launched >= j.total || nseg >= kMaxJobs) continue;
     const uint64_t avail = cap > reservedTotal ? cap - reservedTotal : 0;
@@ -1810,7 +1826,9 @@ void run_generation(monsel_ctx_s* C) {
     j->generations += 1;
     j->stats.transport_ms += kms;
     j->stats.transport_launches += 1;
-    if (j->outstanding == 0) j->reserved = 0;
+    /* an electron still queued has at most K descendants (its energy is below
+     * the beam energy), so the reservation shrinks with the queue */
+    j->reserved = std::min<uint64_t>(j->reserved, (uint64_t)j->outstanding * j->K);
     if (j->generations > maxGen && !C->jobrec[j->slot].killed) {
       /* generation limit: the job's remaining electrons are dropped and counted */
       C->jobrec[j->slot].killed = 1;
@@ -1950,8 +1968,8 @@ int monsel_create(int device_ordinal, const monsel_tuning* tuning, monsel_ctx* o
     HIP_TRY(hipHostMalloc((void**)&C->host_count, (4 + kMaxJobs) * sizeof(unsigned int), hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void**)&C->host_segs, 3 * kMaxJobs * sizeof(uint32_t), hipHostMallocDefault));
     if (C->tuning.max_jobs_in_flight < 0 || C->tuning.primaries_per_launch < 0 || C->tuning.steps_per_launch < 0 ||
-        C->tuning.queue_capacity < 0)
-      throw MonselError(MONSEL_E_INVALID, "negative tuning value");
+        C->tuning.queue_capacity < 0 || C->tuning.admission_order < 0 || C->tuning.admission_order > 1)
+      throw MonselError(MONSEL_E_INVALID, "tuning value out of range");
     C->jobtab.ensure(kMaxJobs * sizeof(JobRec));
     C->counts.ensure((4 + kMaxJobs) * sizeof(unsigned int));
     C->segs.ensure(3 * kMaxJobs * sizeof(uint32_t));

@@ -174,9 +174,10 @@ class Admission:
         self.inflight -= 1
 
 
-def serve(engine, requests, plain=False, hbm_budget=64 << 30, time_scale=1.0, poll_s=0.0005):
+def serve(engine, requests, plain=False, hbm_budget=64 << 30, time_scale=1.0, poll_s=0.0005, log=None):
     """Replays the request stream in real time (arrival times divided by
-    time_scale) and returns per-request latencies and totals."""
+    time_scale) and returns per-request latencies and totals; log(msg), if
+    given, hears of every completed request."""
     cache = SceneCache(engine)
     for r in requests:
         _request_scene(cache, r)
@@ -211,6 +212,9 @@ def serve(engine, requests, plain=False, hbm_budget=64 << 30, time_scale=1.0, po
             adm.release(need)
             lat[r.rid] = (time.perf_counter() - t0) * time_scale - r.arrival
             done += 1
+            if log:
+                log(f"request {r.rid} ({r.side}x{r.side}, {r.per_pixel}/px, {r.beam_eV:.0f} eV, {r.geometry}) "
+                    f"done, latency {lat[r.rid]:.2f} s ({done}/{len(requests)})")
         if not finished:
             if not running and not waiting and pending:
                 time.sleep(max(0.0, (pending[0].arrival - now) / time_scale))
