@@ -261,47 +261,42 @@ MD float mott_cos(const DevScene& S, int mott, float EeV, float u) {
 
 /* ------------------------------------------------------------- CSD ----- */
 
-/* JoyLuoNieminenCSD::compute (JoyLuoNieminenCSD.cu:91-121) for one segment;
- * returns false when the Joy-Luo loss exceeds 10% and the segment must be
- * halved. */
-MD bool csd_segment(const DCSD& c, float len, float kE, float& dE) {
-  dE = 0.f;
-  if (c.nce == 0 || kE < c.minE || kE <= 0.f) return true;
-  if (kE <= c.breakE) {
-    /* kE/(1 + 1.5 gamma len kE^1.5)^(2/3) - kE, written with log1p/expm1 */
-    float eps = 1.5f * c.gamma * len * kE * __builtin_sqrtf(kE);
-    dE = kE * expm1f(-(2.f / 3.f) * log1pf(eps));
-    return true;
-  }
-  float loss = 0.f;
-  for (int i = 0; i < c.nce; ++i) loss += c.coef[i] * __logf(c.recipJ[i] * kE + c.beta[i]);
-  loss *= len / kE;
-  if (loss <= 0.1f * kE) {
-    dE = -loss;
-    return true;
-  }
-  return false;
+/* (1 + eps)^(-2/3) - 1 for eps >= 0: the Taylor series below 1e-3 (error
+ * < 5e-13 relative), the hardware log2/exp2 above (relative error of the
+ * result < 1e-4 there, where it is at least 6.7e-4 in size). */
+MD float nieminen_factor(float eps) {
+  if (eps < 1e-3f) return eps * (-2.f / 3.f + eps * (5.f / 9.f - eps * (40.f / 81.f)));
+  return __builtin_amdgcn_exp2f(-(2.f / 3.f) * __builtin_amdgcn_logf(1.f + eps)) - 1.f;
 }
 
-/* The recursive halving of JoyLuoNieminenCSD.cu:154-159 as an explicit
- * depth-first walk: bit d of `pending` marks a second half at depth d. */
+/* JoyLuoNieminenCSD::compute (JoyLuoNieminenCSD.cu:91-121) with the
+ * recursive halving of :154-159 as an explicit depth-first walk: bit d of
+ * `pending` marks a second half at depth d.  Below breakE the Nieminen form
+ * kE (1 + 1.5 gamma len kE^1.5)^(-2/3) - kE; above it the Joy-Luo loss,
+ * halved while it exceeds 10% of the energy. */
 MD float csd_compute(const DCSD& c, float len, float kE) {
-  if (!c.active) return 0.f;
+  if (!c.active || c.nce == 0) return 0.f;
   float E = kE, total = 0.f;
   uint32_t pending = 0;
   int d = 0;
   for (int it = 0; it < 128; ++it) {
-    float seg = ldexpf(len, -d);
-    float dE;
-    if (!csd_segment(c, seg, E, dE) && d < 30) {
-      pending |= 1u << (d + 1);
-      d += 1;
-      continue;
-    }
-    if (!csd_segment(c, seg, E, dE)) {
-      float loss = 0.f;
-      for (int i = 0; i < c.nce; ++i) loss += c.coef[i] * __logf(c.recipJ[i] * E + c.beta[i]);
-      dE = -loss * seg / E;
+    const float seg = ldexpf(len, -d);
+    float dE = 0.f;
+    if (!(E < c.minE || E <= 0.f)) {
+      if (E <= c.breakE) {
+        const float eps = 1.5f * c.gamma * seg * E * __builtin_sqrtf(E);
+        dE = E * nieminen_factor(eps);
+      } else {
+        float s = 0.f;
+        for (int i = 0; i < c.nce; ++i) s += c.coef[i] * __logf(c.recipJ[i] * E + c.beta[i]);
+        const float loss = s * seg / E;
+        if (loss > 0.1f * E && d < 30) {
+          pending |= 1u << (d + 1);
+          d += 1;
+          continue;
+        }
+        dE = -loss;
+      }
     }
     E += dE;
     total += dE;
