@@ -42,6 +42,19 @@ def build_scene():
     return sc, centers
 
 
+def profile_figures():
+    """HBM traffic per k_transport launch and VALU figures measured with
+    rocprofv3 counter runs of this workload (profiles/k_transport_linescan.md)."""
+    import re
+    path = os.path.join(ROOT, "profiles", "k_transport_linescan.md")
+    if not os.path.exists(path):
+        return {}
+    m = re.search(r"<!-- bench: (.*?) -->", open(path).read())
+    if not m:
+        return {}
+    return {k: float(v) for k, v in (kv.split("=") for kv in m.group(1).split())}
+
+
 def log(msg):
     """Progress on stderr (stdout carries only the JSON line)."""
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -172,6 +185,7 @@ def main():
     launches = sum(s.transport_launches for s in stats)
     # kernel time is zero when event timing is switched off (counter sessions)
     achieved = BYTES_PER_STEP * steps_total / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+    prof = profile_figures()
     se = r.se.sum() / n_prim
     bse = (r.back.sum() - r.se.sum()) / n_prim
     out = {
@@ -207,7 +221,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": int(prof["traffic_bytes_per_launch"]) if "traffic_bytes_per_launch" in prof else None,
+            "traffic_source": "profiles/k_transport_linescan.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+            "valu_busy": prof.get("valu_busy"),
+            "lane_utilisation": prof.get("lane_utilisation"),
             "kernel": "k_transport",
             "kernel_ms_per_launch": round(k_ms / max(launches, 1), 3),
             "bytes_per_step": BYTES_PER_STEP,
