@@ -45,7 +45,7 @@ class Tuning(ctypes.Structure):
         ("block_size", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32), ("max_generations", ctypes.c_int32),
         ("queue_capacity", ctypes.c_int32), ("max_jobs_in_flight", ctypes.c_int32),
         ("primaries_per_launch", ctypes.c_int32), ("steps_per_launch", ctypes.c_int32),
-        ("admission_order", ctypes.c_int32),
+        ("admission_order", ctypes.c_int32), ("sort_queue", ctypes.c_int32),
     ]
 
 

@@ -699,6 +699,101 @@ __global__ void __launch_bounds__(256) k_count_jobs(const uint32_t* job_field, c
     if (lc[i]) atomicAdd(&per_job[i], lc[i]);
 }
 
+/* Counting sort of a launch's output queue by (region, energy class), so
+ * that the waves of the next launch start on electrons of one material and
+ * one energy regime (the cross-section and slowing-down branches switch at
+ * 50 eV and 45 eV).  The order inside a bucket is arbitrary: results do not
+ * depend on it (each electron carries its own random stream). */
+constexpr int kSortKeys = 4096;
+constexpr float kSortSplitEV = 50.f;
+
+__device__ __forceinline__ uint32_t sort_key(const uint32_t* in_u, const float* in_f, uint32_t cap, uint32_t i,
+                                             int n_keys) {
+  const uint32_t k = 2u * in_u[i] + (in_f[6 * (size_t)cap + i] > kSortSplitEV ? 1u : 0u);
+  return k < (uint32_t)n_keys ? k : (uint32_t)n_keys - 1u;
+}
+
+__global__ void __launch_bounds__(256) k_sort_count(const float* in_f, const uint32_t* in_u, uint32_t cap,
+                                                    const unsigned int* count, unsigned int* hist, int n_keys) {
+  __shared__ unsigned int lc[kSortKeys];
+  for (int i = threadIdx.x; i < n_keys; i += blockDim.x) lc[i] = 0u;
+  __syncthreads();
+  const uint32_t n = min(*count, cap);
+  const uint32_t stride = gridDim.x * blockDim.x;
+  /* uniform trip count per wave so that the ballots below see every lane */
+  for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
+    const uint32_t i = base + lane_id();
+    const bool valid = i < n;
+    const uint32_t key = valid ? sort_key(in_u, in_f, cap, i, n_keys) : 0xFFFFFFFFu;
+    /* one LDS atomic per distinct key of the wave */
+    unsigned long long left = wave_ballot(valid);
+    while (left) {
+      const uint32_t k0 = __shfl(key, __ffsll((long long)left) - 1);
+      const unsigned long long same = wave_ballot(key == k0);
+      if (lane_id() == __ffsll((long long)same) - 1) atomicAdd(&lc[k0], (unsigned int)__popcll(same));
+      left &= ~same;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n_keys; i += blockDim.x)
+    if (lc[i]) atomicAdd(&hist[i], lc[i]);
+}
+
+/* exclusive prefix sum of hist[0..n_keys) into offs (one block) */
+__global__ void __launch_bounds__(1024) k_sort_offsets(const unsigned int* hist, unsigned int* offs, int n_keys) {
+  __shared__ unsigned int part[1024];
+  const int per = (n_keys + 1023) / 1024;
+  const int t = threadIdx.x;
+  unsigned int s = 0;
+  for (int k = t * per; k < min(n_keys, (t + 1) * per); ++k) s += hist[k];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const unsigned int v = t >= off ? part[t - off] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  unsigned int run = t > 0 ? part[t - 1] : 0u;
+  for (int k = t * per; k < min(n_keys, (t + 1) * per); ++k) {
+    offs[k] = run;
+    run += hist[k];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_sort_scatter(const float* in_f, const uint32_t* in_u, float* out_f,
+                                                      uint32_t* out_u, uint32_t cap, const unsigned int* count,
+                                                      unsigned int* offs, int n_keys) {
+  const uint32_t n = min(*count, cap);
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const unsigned long long lanemask_lt = (1ull << lane_id()) - 1ull;
+  for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
+    const uint32_t i = base + lane_id();
+    const bool valid = i < n;
+    const uint32_t key = valid ? sort_key(in_u, in_f, cap, i, n_keys) : 0xFFFFFFFFu;
+    /* one global atomic per distinct key of the wave; lanes of a key take
+     * consecutive slots in lane order */
+    uint32_t slot = 0;
+    unsigned long long left = wave_ballot(valid);
+    while (left) {
+      const uint32_t k0 = __shfl(key, __ffsll((long long)left) - 1);
+      const unsigned long long same = wave_ballot(key == k0);
+      const int leader = __ffsll((long long)same) - 1;
+      uint32_t b = 0;
+      if (lane_id() == leader) b = atomicAdd(&offs[k0], (unsigned int)__popcll(same));
+      b = __shfl(b, leader);
+      if (key == k0) slot = b + (uint32_t)__popcll(same & lanemask_lt);
+      left &= ~same;
+    }
+    if (valid) {
+#pragma unroll
+      for (int f = 0; f < kQF; ++f) out_f[(size_t)f * cap + slot] = in_f[(size_t)f * cap + i];
+#pragma unroll
+      for (int f = 0; f < kQU; ++f) out_u[(size_t)f * cap + slot] = in_u[(size_t)f * cap + i];
+    }
+  }
+}
+
 /* Per-primary tallies of one job -> per-pixel sums (exact u64 counts,
  * second moments and fixed-point energy).  px holds back, fwd, se, se^2,
  * bse^2, energy ([6][n_pixels]).  Primaries of one pixel are consecutive, so a wave whose
@@ -1754,7 +1849,7 @@ void run_generation(monsel_ctx_s* C) {
   }
   unsigned int* counts = C->counts.as<unsigned int>();
   /* counts[0] claim, [1] queued electrons, [2] escapes, [4..] per-job queue counts */
-  HIP_TRY(hipMemsetAsync(counts, 0, (4 + kMaxJobs) * sizeof(unsigned int), C->stream));
+  HIP_TRY(hipMemsetAsync(counts, 0, (4 + kMaxJobs) * sizeof(unsigned int), C->stream));  /* sort scratch follows */
   if (nseg > 0)
     HIP_TRY(hipMemcpyAsync(C->segs.p, C->host_segs, 3 * kMaxJobs * 4, hipMemcpyHostToDevice, C->stream));
   const int cur = C->cur;
@@ -1819,6 +1914,29 @@ void run_generation(monsel_ctx_s* C) {
   const uint32_t outc = C->host_count[1];
   C->in_count = outc > cap ? cap : outc;
   C->cur = cur ^ 1;
+  /* regroup the queue by (region, energy class) for the next launch */
+  const int n_keys = 2 * C->batch.dev.n_regions;
+  if (C->tuning.sort_queue >= 0 && C->in_count >= 65536 && n_keys <= kSortKeys) {
+    unsigned int* hist = counts + 4 + kMaxJobs;
+    unsigned int* offs = hist + kSortKeys;
+    const float* sf = C->q[C->cur].as<float>();
+    const uint32_t* su = (const uint32_t*)(sf + (size_t)kQF * cap);
+    float* df = C->q[C->cur ^ 1].as<float>();
+    uint32_t* du = (uint32_t*)(df + (size_t)kQF * cap);
+    HIP_TRY(hipMemsetAsync(hist, 0, (size_t)n_keys * sizeof(unsigned int), C->stream));
+    g_hip_call = "launch k_sort_count";
+    hipLaunchKernelGGL(k_sort_count, dim3(C->n_cu * 2), dim3(256), 0, C->stream, sf, su, cap,
+                       (const unsigned int*)(counts + 1), hist, n_keys);
+    HIP_TRY(hipGetLastError());
+    g_hip_call = "launch k_sort_offsets";
+    hipLaunchKernelGGL(k_sort_offsets, dim3(1), dim3(1024), 0, C->stream, (const unsigned int*)hist, offs, n_keys);
+    HIP_TRY(hipGetLastError());
+    g_hip_call = "launch k_sort_scatter";
+    hipLaunchKernelGGL(k_sort_scatter, dim3(C->n_cu * 8), dim3(256), 0, C->stream, sf, su, df, du, cap,
+                       (const unsigned int*)(counts + 1), offs, n_keys);
+    HIP_TRY(hipGetLastError());
+    C->cur ^= 1;
+  }
   float kms = 0.f;
   if (timing) HIP_TRY(hipEventElapsedTime(&kms, C->evk0, C->evk1));
   for (Job* j : inflight) {
@@ -1968,10 +2086,11 @@ int monsel_create(int device_ordinal, const monsel_tuning* tuning, monsel_ctx* o
     HIP_TRY(hipHostMalloc((void**)&C->host_count, (4 + kMaxJobs) * sizeof(unsigned int), hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void**)&C->host_segs, 3 * kMaxJobs * sizeof(uint32_t), hipHostMallocDefault));
     if (C->tuning.max_jobs_in_flight < 0 || C->tuning.primaries_per_launch < 0 || C->tuning.steps_per_launch < 0 ||
-        C->tuning.queue_capacity < 0 || C->tuning.admission_order < 0 || C->tuning.admission_order > 1)
+        C->tuning.queue_capacity < 0 || C->tuning.admission_order < 0 || C->tuning.admission_order > 1 ||
+        C->tuning.sort_queue < -1 || C->tuning.sort_queue > 0)
       throw MonselError(MONSEL_E_INVALID, "tuning value out of range");
     C->jobtab.ensure(kMaxJobs * sizeof(JobRec));
-    C->counts.ensure((4 + kMaxJobs) * sizeof(unsigned int));
+    C->counts.ensure((4 + kMaxJobs + 2 * kSortKeys) * sizeof(unsigned int));
     C->segs.ensure(3 * kMaxJobs * sizeof(uint32_t));
     for (int k = kMaxJobs - 1; k >= 0; --k) C->free_slots.push_back(k);
     C->qcap = queue_capacity(C.get());

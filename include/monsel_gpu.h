@@ -275,6 +275,7 @@ typedef struct monsel_tuning {
   int32_t primaries_per_launch; /* new primaries admitted per launch (0 = queue capacity) */
   int32_t steps_per_launch;     /* regroup interval: steps an electron takes per launch (0 = unlimited) */
   int32_t admission_order;      /* new primaries go first to: 0 the oldest job, 1 the job with fewest left */
+  int32_t sort_queue;           /* 0: sort each launch's queue by region and energy class; -1: keep spawn order */
 } monsel_tuning;
 
 typedef struct monsel_ctx_s* monsel_ctx;
