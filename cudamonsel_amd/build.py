@@ -5,6 +5,9 @@
   _monsel*.so        g++ + pybind11: Python bindings of the host API
 
 Usage: python -m cudamonsel_amd.build   (or build() from Python)
+       python -m cudamonsel_amd.build --phase-timing
+           the same with the transport kernel's per-phase wave timing
+           compiled in (csrc/phase_timing.h); for profiling only
 """
 import os
 import subprocess
@@ -43,11 +46,12 @@ def ext_suffix():
     return sysconfig.get_config_var("EXT_SUFFIX") or ".so"
 
 
-def build(verbose=True):
+def build(verbose=True, phase_timing=False):
     gpu = os.path.join(HERE, "libmonsel_gpu.so")
+    extra = ["-DMONSEL_PHASE_TIMING"] if phase_timing else []
     host = os.path.join(HERE, "libmonsel_host.so")
     mod = os.path.join(HERE, "_monsel" + ext_suffix())
-    _run([os.path.join(ROCM, "bin", "hipcc"), *HIPCC_FLAGS,
+    _run([os.path.join(ROCM, "bin", "hipcc"), *HIPCC_FLAGS, *extra,
           os.path.join(HERE, "csrc", "monsel_gpu.hip"), "-o", gpu], verbose)
     _run(["g++", *CXX_FLAGS, os.path.join(HERE, "host", "monsel_host.cpp"),
           "-L" + HERE, "-lmonsel_gpu", "-Wl,-rpath,$ORIGIN", "-ldl", "-o", host], verbose)
@@ -60,4 +64,4 @@ def build(verbose=True):
 
 
 if __name__ == "__main__":
-    build(verbose="-q" not in sys.argv)
+    build(verbose="-q" not in sys.argv, phase_timing="--phase-timing" in sys.argv)

@@ -10,6 +10,17 @@
 
 #include <stdint.h>
 
+/* Scene records are read-only for the lifetime of a kernel, so device code
+ * reads them through the constant address space: a load whose index is the
+ * same on every lane of a wave then becomes one scalar load into SGPRs
+ * instead of a vector load per lane.  (The host pass sees plain pointers of
+ * the same size.) */
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MONSEL_CS __attribute__((address_space(4)))
+#else
+#define MONSEL_CS
+#endif
+
 namespace monsel_dev {
 
 constexpr int kMaxPrims = 8;      /* convex primitives per region shape */
@@ -27,7 +38,7 @@ struct DPrim {
   int kind;
   int first_plane;
   int n_planes;
-  int pad;
+  float sdscale; /* planes: min over the planes of 1/|n| (distance bounds); 1 otherwise */
   float e0[3];   /* cylinder end0 / sphere centre (nm) */
   float ax[3];   /* cylinder unit axis */
   float len;     /* cylinder length (nm) */
@@ -126,22 +137,23 @@ struct DNUTable {
 
 struct DevScene {
   int n_regions, n_shapes, n_prims, n_planes, n_msm, n_mech, n_elem, n_mott, n_nu;
+  int has_tabulated;      /* some mechanism is TabulatedInelasticSM (selects the transport kernel) */
   uint32_t compat;
   float chamber_r;        /* nm */
   float chamber_c[3];     /* nm */
-  const DRegion* regions;
-  const int* children;
-  const DShape* shapes;
-  const DPrim* prims;
-  const float4* planes;   /* (nx, ny, nz, b) */
-  const DMSM* msms;
-  const DMech* mechs;
-  const DElem* elems;
-  const float* mott_spwem; /* [n_mott][61] */
-  const float* mott_x1;    /* [n_mott][61*201] */
-  const DNUTable* nu;
-  const float* nu_data;
-  const double* nu_data64; /* the same values and abscissae in FP64 */
+  const MONSEL_CS DRegion* regions;
+  const MONSEL_CS int* children;
+  const MONSEL_CS DShape* shapes;
+  const MONSEL_CS DPrim* prims;
+  const MONSEL_CS float4* planes;   /* (nx, ny, nz, b) */
+  const MONSEL_CS DMSM* msms;
+  const MONSEL_CS DMech* mechs;
+  const MONSEL_CS DElem* elems;
+  const MONSEL_CS float* mott_spwem; /* [n_mott][61] */
+  const MONSEL_CS float* mott_x1;    /* [n_mott][61*201] */
+  const MONSEL_CS DNUTable* nu;
+  const MONSEL_CS float* nu_data;
+  const MONSEL_CS double* nu_data64; /* the same values and abscissae in FP64 */
 };
 
 }  // namespace monsel_dev

@@ -111,6 +111,9 @@ constexpr int kWatchdogLog = 16, kWatchdogFields = 10;
 
 /* One transport launch ("generation"): new primaries described by
  * segments, then the queued electrons of the previous launch. */
+/* lanes of a wave waiting for findEndOfStep before the wave runs it */
+constexpr int kDefaultGeometryBatch = 1;
+
 struct GenDev {
   const JobRec* jobs;
   const uint32_t* seg_end;   /* [n_seg] running item count at the end of each segment */
@@ -133,11 +136,17 @@ struct GenDev {
   uint32_t esc_cap;
   unsigned int* esc_count;
   int steps_per_launch;      /* regroup interval: steps before an electron returns to the queue (0 = none) */
+  int geo_batch;             /* lanes of a wave that wait for findEndOfStep before the wave runs it */
 };
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
 __device__ __forceinline__ unsigned long long wave_ballot(bool p) { return __ballot(p); }
+
+/* this lane's bit of a ballot.  Waterfall loops test membership this way
+ * rather than by comparing with the readfirstlane value, which would let the
+ * compiler substitute the per-lane value for the uniform one in the body. */
+__device__ __forceinline__ bool lane_in(unsigned long long m) { return (m >> lane_id()) & 1ull; }
 
 /* Histogram::bin with edges min + i*delta (Histogram.cu:73-79,
  * Algorithm.cu:5-14); compat = the reference's insertion-point semantics. */
@@ -219,7 +228,7 @@ enum StepResult { STEP_CONTINUE = 0, STEP_DEAD = 1, STEP_ESCAPED = 2, STEP_WATCH
  * `next`, given the outward barrier normal (nbx, nby, nbz) resolved from the
  * struck shape.  Moves the electron a nudge along +nb (returns +1) or -nb
  * (returns -1); u1 decides a quantum-mechanical transmission. */
-__device__ __forceinline__ int barrier_core(const DevScene& S, const DMSM& msm, const DMSM& nmsm, Elec& e, int next,
+__device__ __forceinline__ int barrier_core(const DevScene& S, const MONSEL_CS DMSM& msm, const MONSEL_CS DMSM& nmsm, Elec& e, int next,
                                             float nbx, float nby, float nbz, float u1) {
   const float deltaU = -msm.cb + nmsm.cb;
   const bool compat = (S.compat & MONSEL_COMPAT_BARRIER_NORMAL) != 0;
@@ -289,22 +298,32 @@ __device__ __forceinline__ int barrier_core(const DevScene& S, const DMSM& msm, 
   return -1;
 }
 
-/* One MonteCarloSS::takeStep (MonteCarloSS.cu:107-163). */
-__device__ __forceinline__ int take_step(const DevScene& S, const JobRec* jobs, Elec& e, Spawn& sp, int maxSteps) {
-  sp.has = false;
-  int region = e.region;
-  if (region < 0 || !shape_contains(S, S.regions[region].shape, e.px, e.py, e.pz)) {
-    region = containing_sub_region(S, jobs[e.job].root, e.px, e.py, e.pz);
-    e.region = region;
-    if (region < 0) return STEP_DEAD;
-  }
-  const DMSM& msm = S.msms[S.regions[region].msm];
-  float u[kStepUniforms];
+/* Per-lane rates of the step in progress (mechanism k of the region's scatter
+ * model), kept between the step's start and its end. */
+__device__ __forceinline__ float* lane_rates() {
+  __shared__ float l_rate[kMaxMech * kLaneBlock];
+  return l_rate + threadIdx.x;
+}
+
+/* What the start of a step leaves for its end. */
+struct StepStart {
+  float freePath;
+  float total;   /* summed mechanism rates (0 in a null material) */
+  float u1, u2, u3;
+  bool geo;      /* the step may reach a boundary: findEndOfStep must run */
+};
+
+/* MonteCarloSS::takeStep (MonteCarloSS.cu:107-163), first half: the
+ * mechanism rates and the free path (MONSEL_MaterialScatterModel::
+ * randomMeanPathLength, :64-91).  The electron is located (its region
+ * contains it).  A free path that ends at least a nudge short of e.safe
+ * cannot reach a boundary; every other step needs the geometry. */
+__device__ __forceinline__ void begin_step_in(const DevScene& S, int region, const Elec& e, StepStart& st) {
+  PT_BEGIN(t_ra);
+  const MONSEL_CS DMSM& msm = S.msms[S.regions[region].msm];
+  float u[4];
   step_uniforms_lo(e, kPhiloxKey0, kPhiloxKey1, u);
-  /* MONSEL_MaterialScatterModel::randomMeanPathLength (:64-91); each
-   * mechanism's rate is kept in a per-lane LDS slot for the scatter choice */
-  __shared__ float l_rate[kMaxMech][kLaneBlock];
-  const int tid = threadIdx.x;
+  float* rates = lane_rates();
   float total = 0.f;
   float freePath;
   const float maxFreePath = 2.f * S.chamber_r;
@@ -313,7 +332,7 @@ __device__ __forceinline__ int take_step(const DevScene& S, const JobRec* jobs, 
   } else {
     for (int k = 0; k < msm.n_mech; ++k) {
       const float rk = mech_rate(S, S.mechs[msm.mech0 + k], e.E);
-      l_rate[k][tid] = rk;
+      rates[k * kLaneBlock] = rk;
       total += rk;
     }
     if (total != 0.f) {
@@ -323,12 +342,71 @@ __device__ __forceinline__ int take_step(const DevScene& S, const JobRec* jobs, 
       freePath = maxFreePath;
     }
   }
+  st.freePath = freePath;
+  st.total = total;
+  st.u1 = u[1];
+  st.u2 = u[2];
+  st.u3 = u[3];
+  st.geo = !(freePath + nudge_nm(e.px, e.py, e.pz) <= e.safe);
+  PT_END(PH_RATES, t_ra);
+}
+
+/* The lanes of a wave take their turn by region (one pass per region present,
+ * normally one): within a pass the region index is wave-uniform, so the
+ * scene records it selects are read with scalar loads. */
+__device__ __forceinline__ void begin_step(const DevScene& S, const Elec& e, StepStart& st) {
+  PT_UNIFORM(PH_RATES_UNI, e.region);
+  for (;;) {
+    const int r0 = __builtin_amdgcn_readfirstlane(e.region);
+    if (lane_in(wave_ballot(e.region == r0))) {
+      begin_step_in(S, r0, e, st);
+      break;
+    }
+  }
+}
+
+/* The check that opens MonteCarloSS::takeStep (MonteCarloSS.cu:107-115):
+ * re-locates an electron whose region (r == e.region, passed separately
+ * where it is wave-uniform) does not contain it.  Returns false when no
+ * region does. */
+__device__ __forceinline__ bool locate(const DevScene& S, const JobRec* jobs, Elec& e, int r) {
+  PT_BEGIN(t_co);
+  e.safe = 0.f;
+  if (r < 0 || !shape_contains(S, S.regions[r].shape, e.px, e.py, e.pz))
+    e.region = containing_sub_region(S, jobs[e.job].root, e.px, e.py, e.pz);
+  PT_END(PH_CONTAIN, t_co);
+  return e.region >= 0;
+}
+
+/* Second half of takeStep: the end of the step (findEndOfStep when st.geo),
+ * the move, slowing down, and scatter, barrier crossing or escape.  Ends
+ * with the containment check of the next step's start wherever the electron
+ * may have left its region (a boundary step, or one that ended within a
+ * nudge of a surface). */
+template <bool TAB>
+__device__ __forceinline__ int finish_step_in(const DevScene& S, const JobRec* jobs, const int region, Elec& e,
+                                              Spawn& sp, int maxSteps, const StepStart& st) {
+  sp.has = false;
+  const MONSEL_CS DMSM& msm = S.msms[S.regions[region].msm];
+  const float freePath = st.freePath;
   const float px = e.px, py = e.py, pz = e.pz;
-  StepEnd se = find_end_of_step(S, region, px, py, pz, e.dx, e.dy, e.dz, freePath);
+  StepEnd se;
+  if (st.geo) {
+    PT_BEGIN(t_ge);
+    se = find_end_of_step(S, region, px, py, pz, e.dx, e.dy, e.dz, freePath);
+    PT_END(PH_GEOM, t_ge);
+  } else {
+    se.t = freePath;
+    se.next = region;
+    se.boundary = false;
+    se.winner = -1;
+    se.safe = e.safe - freePath;
+  }
   float len = se.t;
   e.px = px + len * e.dx;
   e.py = py + len * e.dy;
   e.pz = pz + len * e.dz;
+  e.safe = se.boundary ? 0.f : se.safe;
   if (se.boundary && se.next == region) {
     /* a grazing crossing: the point a nudge beyond the boundary is back in
      * this region (a corner clipped within the nudge, or FP32 rounding
@@ -341,36 +419,49 @@ __device__ __forceinline__ int take_step(const DevScene& S, const JobRec* jobs, 
     e.pz += dd * e.dz;
   }
   float prevE = e.E;
+  PT_BEGIN(t_cs);
   if (!msm.is_null) e.E += csd_compute(msm.csd, len, e.E);
+  PT_END(PH_CSD, t_cs);
   e.steps++;
   if ((int)e.steps > maxSteps || !(e.E == e.E)) return STEP_WATCHDOG;
   if (e.E < msm.minE) return STEP_DEAD;
   if (se.next == region) {
     /* MONSEL_MaterialScatterModel::scatter (:93-134), rates at prevE */
-    if (msm.is_null || total == 0.f) return STEP_CONTINUE;
-    const float r = u[1] * total;
-    /* first mechanism whose cumulative rate reaches r (MONSEL_MaterialScatterModel.cu:129-131);
-     * the running sum repeats the order of `total` */
-    int idx = 0;
-    float rate = l_rate[0][tid];
-    float cum = rate;
-    while (idx < msm.n_mech - 1 && cum < r) {
-      ++idx;
-      rate = l_rate[idx][tid];
-      cum += rate;
+    if (!msm.is_null && st.total != 0.f) {
+      PT_BEGIN(t_se);
+      const float* rates = lane_rates();
+      const float r = st.u1 * st.total;
+      /* first mechanism whose cumulative rate reaches r (MONSEL_MaterialScatterModel.cu:129-131);
+       * the running sum repeats the order of `total` */
+      int idx = 0;
+      float rate = rates[0];
+      float cum = rate;
+      while (idx < msm.n_mech - 1 && cum < r) {
+        ++idx;
+        rate = rates[idx * kLaneBlock];
+        cum += rate;
+      }
+      bool trapped = false;
+      float u[kStepUniforms];
+      u[0] = 0.f;
+      u[1] = st.u1;
+      u[2] = st.u2;
+      u[3] = st.u3;
+      step_uniforms_hi(e, kPhiloxKey0, kPhiloxKey1, u);
+      PT_END(PH_SELECT, t_se);
+      sp = mech_scatter<TAB>(S, S.mechs[msm.mech0 + idx], e, prevE, rate, trapped, u);
+      if (sp.has) {
+        /* trackSecondaryElectron (MonteCarloSS.cu:165-174) */
+        if (!(sp.E > msm.minE)) sp.has = false;
+      }
+      if (trapped || e.E < msm.minE) return STEP_DEAD;
     }
-    bool trapped = false;
-    step_uniforms_hi(e, kPhiloxKey0, kPhiloxKey1, u);
-    sp = mech_scatter(S, S.mechs[msm.mech0 + idx], e, prevE, rate, trapped, u);
-    if (sp.has) {
-      /* trackSecondaryElectron (MonteCarloSS.cu:165-174) */
-      if (!(sp.E > msm.minE)) sp.has = false;
-    }
-    if (trapped || e.E < msm.minE) return STEP_DEAD;
+    if (!(e.safe > nudge_nm(e.px, e.py, e.pz)) && !locate(S, jobs, e, region)) return STEP_DEAD;
     return STEP_CONTINUE;
   }
   if (se.next >= 0) {
     /* ExpQMBarrierSM::barrierScatter (ExpQMBarrierSM.cu:50-219) */
+    PT_BEGIN(t_ba);
     int next = se.next;
     if (msm.is_null || msm.barrier_kind == MONSEL_BARRIER_NONE) {
       e.region = next;
@@ -392,7 +483,7 @@ __device__ __forceinline__ int take_step(const DevScene& S, const JobRec* jobs, 
           nbx = se.ownNx; nby = se.ownNy; nbz = se.ownNz;
         }
       }
-      barrier_core(S, msm, S.msms[S.regions[next].msm], e, next, nbx, nby, nbz, u[1]);
+      barrier_core(S, msm, S.msms[S.regions[next].msm], e, next, nbx, nby, nbz, st.u1);
     }
     /* MonteCarloSS.cu:146-148: nudge along the new direction; the region is
      * re-located by the check that opens the next step */
@@ -400,6 +491,8 @@ __device__ __forceinline__ int take_step(const DevScene& S, const JobRec* jobs, 
     e.px += dd * e.dx;
     e.py += dd * e.dy;
     e.pz += dd * e.dz;
+    PT_END(PH_BARRIER, t_ba);
+    if (!locate(S, jobs, e, e.region)) return STEP_DEAD;
     return STEP_CONTINUE;
   }
   /* left the chamber: BackscatterEvent.  The escape is recorded for
@@ -414,6 +507,22 @@ __device__ __forceinline__ int take_step(const DevScene& S, const JobRec* jobs, 
   }
   e.region = fromRoot ? 1 : 0;
   return STEP_ESCAPED;
+}
+
+/* finish_step_in by region, as begin_step */
+template <bool TAB>
+__device__ __forceinline__ int finish_step(const DevScene& S, const JobRec* jobs, Elec& e, Spawn& sp, int maxSteps,
+                                           const StepStart& st) {
+  PT_UNIFORM(PH_GEOM_UNI, e.region);
+  int res;
+  for (;;) {
+    const int r0 = __builtin_amdgcn_readfirstlane(e.region);
+    if (lane_in(wave_ballot(e.region == r0))) {
+      res = finish_step_in<TAB>(S, jobs, r0, e, sp, maxSteps, st);
+      break;
+    }
+  }
+  return res;
 }
 
 /* GaussianBeam::createElectron (GaussianBeam.cu:52-65) for primary `idx` of
@@ -440,6 +549,7 @@ __device__ __forceinline__ void make_primary(const DevScene& S, const JobRec& J,
   e.dy = 0.f;
   e.dz = 1.f;
   e.E = J.E0;
+  e.safe = 0.f;
   e.region = containing_sub_region(S, J.root, e.px, e.py, e.pz);
 }
 
@@ -493,17 +603,21 @@ __device__ __forceinline__ void flush_lane_stats(const GenDev& G, uint32_t job, 
 #ifndef MONSEL_TRANSPORT_WAVES_PER_EU
 #define MONSEL_TRANSPORT_WAVES_PER_EU 4
 #endif
+template <bool TAB>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MONSEL_TRANSPORT_WAVES_PER_EU)))
 k_transport(DevScene S, GenDev G) {
   Elec e;
-  bool alive = false, done = false;
+  StepStart st;
+  bool alive = false, done = false, pending = false;
   int maxSteps = 0;
   uint32_t steps0 = 0;                 /* step count when the lane took the electron */
   uint32_t cur_job = 0xFFFFFFFFu;      /* job the lane's counters belong to */
   uint32_t nsteps = 0, nelec = 0, nesc = 0, nwd = 0, nover = 0;
   const uint32_t n_items = G.n_prim + G.in_n;
   const unsigned long long lanemask_lt = (1ull << lane_id()) - 1ull;
+  PT_INIT();
   for (;;) {
+    PT_BEGIN(t_cl);
     /* claim work for idle lanes: one atomic per wave */
     bool need = !alive && !done;
     unsigned long long m = wave_ballot(need);
@@ -543,6 +657,7 @@ k_transport(DevScene S, GenDev G) {
             e.job = G.in_u[4 * cap + q];
             e.steps = G.in_u[5 * cap + q];
             e.nchild = G.in_u[6 * cap + q];
+            e.safe = 0.f;
           }
           if (e.job != cur_job) {
             if (cur_job != 0xFFFFFFFFu) flush_lane_stats(G, cur_job, nsteps, nelec, nesc, nwd, nover);
@@ -554,8 +669,9 @@ k_transport(DevScene S, GenDev G) {
           } else {
             maxSteps = J.max_steps;
             steps0 = e.steps;
-            alive = true;
             if (e.steps == 0) nelec++;
+            /* a queued electron is located again (the check opening its next step) */
+            alive = idx < G.n_prim ? e.region >= 0 : locate(S, G.jobs, e, e.region);
           }
         } else {
           done = true;
@@ -563,13 +679,29 @@ k_transport(DevScene S, GenDev G) {
       }
     }
     if (wave_ballot(alive) == 0ull) break;
+    PT_END(PH_CLAIM, t_cl);
+    /* start a step on every lane without one in progress; a step that may
+     * reach a boundary waits for the wave's next findEndOfStep pass, which
+     * runs once geo_batch lanes wait or no lane could move without it */
+    if (alive && !pending) {
+      begin_step(S, e, st);
+      pending = true;
+    }
+    const unsigned long long m_geo = wave_ballot(alive && st.geo);
+    const unsigned long long m_free = wave_ballot(alive && !st.geo);
+    const bool run_geo = m_free == 0ull || __popcll(m_geo) >= G.geo_batch;
     Spawn sp;
     sp.has = false;
     int res = STEP_CONTINUE;
-    if (alive) {
-      res = take_step(S, G.jobs, e, sp, maxSteps);
+    const bool stepped = alive && (!st.geo || run_geo);
+    if (stepped) {
+      PT_BEGIN(t_st);
+      res = finish_step<TAB>(S, G.jobs, e, sp, maxSteps, st);
+      pending = false;
       nsteps++;
+      PT_END(PH_STEP, t_st);
     }
+    PT_BEGIN(t_ta);
     /* secondaries to the next launch */
     uint32_t clo = 0, chi = 0;
     if (sp.has) {
@@ -580,12 +712,12 @@ k_transport(DevScene S, GenDev G) {
     queue_append(G, sp.has, e.px, e.py, e.pz, sp.dx, sp.dy, sp.dz, sp.E, e.region, e.primary, clo, chi, e.job, 0u, 0u,
                  lanemask_lt, nover);
     /* regroup: an electron past its step budget for this launch continues in the next one */
-    const bool requeue = alive && res == STEP_CONTINUE && G.steps_per_launch > 0 &&
+    const bool requeue = stepped && res == STEP_CONTINUE && G.steps_per_launch > 0 &&
                          (int)(e.steps - steps0) >= G.steps_per_launch;
     queue_append(G, requeue, e.px, e.py, e.pz, e.dx, e.dy, e.dz, e.E, e.region, e.primary, e.lin_lo, e.lin_hi, e.job,
                  e.steps, e.nchild, lanemask_lt, nover);
     /* escape records: one atomic per wave */
-    const bool esc = alive && res == STEP_ESCAPED;
+    const bool esc = stepped && res == STEP_ESCAPED;
     unsigned long long em = wave_ballot(esc);
     if (em) {
       int leader = __ffsll((long long)em) - 1;
@@ -610,7 +742,7 @@ k_transport(DevScene S, GenDev G) {
         }
       }
     }
-    if (alive && (res != STEP_CONTINUE || requeue)) {
+    if (stepped && (res != STEP_CONTINUE || requeue)) {
       alive = false;
       if (res == STEP_ESCAPED) nesc++;
       if (res == STEP_WATCHDOG) {
@@ -627,7 +759,9 @@ k_transport(DevScene S, GenDev G) {
         }
       }
     }
+    PT_END(PH_TAIL, t_ta);
   }
+  PT_FLUSH();
   /* flush counters: reduced across the wave when every lane's counters
    * belong to one job, per lane otherwise */
   const unsigned long long valid = wave_ballot(cur_job != 0xFFFFFFFFu);
@@ -844,7 +978,7 @@ __global__ void k_reduce_primaries(const JobRec* jobs, int slot, uint32_t n, uns
 __global__ void k_probe_rates(DevScene S, int msm, int n, const float* E, float* out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const DMSM& m = S.msms[msm];
+  const MONSEL_CS DMSM& m = S.msms[msm];
   float tot = 0.f;
   for (int k = 0; k < m.n_mech; ++k) {
     float r = mech_rate(S, S.mechs[m.mech0 + k], E[i]);
@@ -858,8 +992,8 @@ __global__ void k_probe_elastic(DevScene S, int mech, int element, int n, const 
                                 float* out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const DMech& m = S.mechs[mech];
-  const DElem& el = S.elems[m.elem0 + element];
+  const MONSEL_CS DMech& m = S.mechs[mech];
+  const MONSEL_CS DElem& el = S.elems[m.elem0 + element];
   float EeV = E[i], r1 = u[2 * i], r2 = u[2 * i + 1];
   float ca;
   if (el.kind == MONSEL_MECH_BROWNING_ELASTIC || (el.kind == MONSEL_MECH_NISTMOTT_ELASTIC && EeV < el.extrap))
@@ -878,7 +1012,7 @@ __global__ void k_probe_scatter(DevScene S, int mech, int n, const float* prevE,
                                 const float* u, float* out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const DMech& m = S.mechs[mech];
+  const MONSEL_CS DMech& m = S.mechs[mech];
   Elec e;
   e.px = e.py = e.pz = 0.f;
   e.dx = dir[3 * i];
@@ -1347,14 +1481,19 @@ HostScene compile_scene(const monsel_scene_desc* d) {
         p.kind = PRIM_PLANES;
         p.first_plane = (int)H.planes.size();
         p.n_planes = sh.n_planes;
+        double maxn = 0.;
         for (int k = 0; k < sh.n_planes; ++k) {
           const monsel_plane_desc& pl = d->planes[sh.first_plane + k];
           double n[3] = {pl.normal[0], pl.normal[1], pl.normal[2]};
           double b = (n[0] * pl.point[0] + n[1] * pl.point[1] + n[2] * pl.point[2]) * 1e9;
           H.planes.push_back(make_float4((float)n[0], (float)n[1], (float)n[2], (float)b));
+          maxn = std::max(maxn, sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]));
         }
+        /* 1/|n| rounded down a little so that FP32 distance bounds stay below the true distance */
+        p.sdscale = maxn > 0. ? (float)(0.999 / maxn) : 0.f;
       } else if (sh.kind == MONSEL_SHAPE_CYLINDER) {
         p.kind = PRIM_CYLINDER;
+        p.sdscale = 1.f;
         double ax[3], L2 = 0;
         for (int k = 0; k < 3; ++k) {
           ax[k] = (sh.end1[k] - sh.end0[k]) * 1e9;
@@ -1372,6 +1511,7 @@ HostScene compile_scene(const monsel_scene_desc* d) {
         p.r2 = (float)(sh.radius * 1e9 * sh.radius * 1e9);
       } else if (sh.kind == MONSEL_SHAPE_SPHERE) {
         p.kind = PRIM_SPHERE;
+        p.sdscale = 1.f;
         for (int k = 0; k < 3; ++k) p.e0[k] = (float)(sh.center[k] * 1e9);
         p.r2 = (float)(sh.radius * 1e9 * sh.radius * 1e9);
         ds.is_sphere = 1;
@@ -1462,22 +1602,25 @@ void upload_scene(const HostScene& H, SceneDev& sd) {
   S.n_elem = (int)H.elems.size();
   S.n_mott = (int)(H.spwem.size() / kMottNE);
   S.n_nu = (int)H.nu.size();
+  S.has_tabulated = 0;
+  for (const DMech& m : H.mechs)
+    if (m.kind == MONSEL_MECH_TABULATED_INELASTIC) S.has_tabulated = 1;
   S.compat = H.compat;
   S.chamber_r = H.chamber_r;
   for (int k = 0; k < 3; ++k) S.chamber_c[k] = H.chamber_c[k];
-  S.regions = (const DRegion*)(b + o_reg);
-  S.children = (const int*)(b + o_ch);
-  S.shapes = (const DShape*)(b + o_sh);
-  S.prims = (const DPrim*)(b + o_pr);
-  S.planes = (const float4*)(b + o_pl);
-  S.msms = (const DMSM*)(b + o_msm);
-  S.mechs = (const DMech*)(b + o_me);
-  S.elems = (const DElem*)(b + o_el);
-  S.mott_spwem = (const float*)(b + o_sp);
-  S.mott_x1 = (const float*)(b + o_x1);
-  S.nu = (const DNUTable*)(b + o_nu);
-  S.nu_data = (const float*)(b + o_nd);
-  S.nu_data64 = (const double*)(b + o_nd64);
+  S.regions = (decltype(S.regions))(b + o_reg);
+  S.children = (decltype(S.children))(b + o_ch);
+  S.shapes = (decltype(S.shapes))(b + o_sh);
+  S.prims = (decltype(S.prims))(b + o_pr);
+  S.planes = (decltype(S.planes))(b + o_pl);
+  S.msms = (decltype(S.msms))(b + o_msm);
+  S.mechs = (decltype(S.mechs))(b + o_me);
+  S.elems = (decltype(S.elems))(b + o_el);
+  S.mott_spwem = (decltype(S.mott_spwem))(b + o_sp);
+  S.mott_x1 = (decltype(S.mott_x1))(b + o_x1);
+  S.nu = (decltype(S.nu))(b + o_nu);
+  S.nu_data = (decltype(S.nu_data))(b + o_nd);
+  S.nu_data64 = (decltype(S.nu_data64))(b + o_nd64);
   sd.bytes = blob.size();
 }
 
@@ -1875,6 +2018,7 @@ void run_generation(monsel_ctx_s* C) {
   G.esc_cap = cap;
   G.esc_count = counts + 2;
   G.steps_per_launch = C->tuning.steps_per_launch;
+  G.geo_batch = C->tuning.geometry_batch > 0 ? C->tuning.geometry_batch : kDefaultGeometryBatch;
   const int block = C->tuning.block_size > 0 ? C->tuning.block_size : 256;
   const int grid = C->n_cu * (C->blocks_per_cu > 0 ? C->blocks_per_cu : 4);
   /* Event markers time the kernels; MONSEL_NO_EVENT_TIMING=1 leaves them out
@@ -1882,7 +2026,11 @@ void run_generation(monsel_ctx_s* C) {
   const bool timing = !getenv("MONSEL_NO_EVENT_TIMING");
   if (timing) HIP_TRY(hipEventRecord(C->evk0, C->stream));
   g_hip_call = "launch k_transport";
-  hipLaunchKernelGGL(k_transport, dim3(grid), dim3(block), 0, C->stream, C->batch.dev, G);
+  /* scenes without a tabulated mechanism run the kernel built without it */
+  if (C->batch.dev.has_tabulated)
+    hipLaunchKernelGGL(k_transport<true>, dim3(grid), dim3(block), 0, C->stream, C->batch.dev, G);
+  else
+    hipLaunchKernelGGL(k_transport<false>, dim3(grid), dim3(block), 0, C->stream, C->batch.dev, G);
   HIP_TRY(hipGetLastError());
   if (timing) HIP_TRY(hipEventRecord(C->evk1, C->stream));
   /* escape tallies: the histograms go through LDS when one job is in flight */
@@ -2087,7 +2235,8 @@ int monsel_create(int device_ordinal, const monsel_tuning* tuning, monsel_ctx* o
     HIP_TRY(hipHostMalloc((void**)&C->host_segs, 3 * kMaxJobs * sizeof(uint32_t), hipHostMallocDefault));
     if (C->tuning.max_jobs_in_flight < 0 || C->tuning.primaries_per_launch < 0 || C->tuning.steps_per_launch < 0 ||
         C->tuning.queue_capacity < 0 || C->tuning.admission_order < 0 || C->tuning.admission_order > 1 ||
-        C->tuning.sort_queue < -1 || C->tuning.sort_queue > 0)
+        C->tuning.sort_queue < -1 || C->tuning.sort_queue > 0 || C->tuning.geometry_batch < 0 ||
+        C->tuning.geometry_batch > 64)
       throw MonselError(MONSEL_E_INVALID, "tuning value out of range");
     C->jobtab.ensure(kMaxJobs * sizeof(JobRec));
     C->counts.ensure((4 + kMaxJobs + 2 * kSortKeys) * sizeof(unsigned int));
@@ -2107,8 +2256,10 @@ int monsel_create(int device_ordinal, const monsel_tuning* tuning, monsel_ctx* o
       throw MonselError(MONSEL_E_INVALID, "block_size must be a multiple of 64, <= 256");
     if (C->tuning.blocks_per_cu > 0) C->blocks_per_cu = C->tuning.blocks_per_cu;
     else {
-      int occ = 0;
-      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_transport, block, 0));
+      int occ_t = 0, occ_f = 0;
+      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_t, k_transport<true>, block, 0));
+      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_f, k_transport<false>, block, 0));
+      const int occ = std::min(occ_t, occ_f);
       C->blocks_per_cu = occ > 0 ? occ : 1;
     }
     monsel_ctx_s* raw = C.get();
@@ -2530,5 +2681,22 @@ int monsel_probe_barrier(monsel_ctx ctx, monsel_scene_id scene, int region_from,
     return MONSEL_OK;
   });
 }
+
+#ifdef MONSEL_PHASE_TIMING
+/* Phase-timing builds only: copies out and clears the transport kernel's
+ * phase counters on `device` (out[3k..3k+2] = cycles, entries, active lanes
+ * of phase k; n >= 3 * kPhases).  Waits for the device first. */
+int monsel_phase_timing_read(int device, uint64_t* out, int n) {
+  if (!out || n < 3 * kPhases) return set_err(MONSEL_E_INVALID, "phase_timing_read arguments");
+  return guarded([&] {
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpyFromSymbol(out, HIP_SYMBOL(g_phase), sizeof(unsigned long long) * 3 * kPhases));
+    std::vector<unsigned long long> zero(3 * kPhases, 0ull);
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(g_phase), zero.data(), sizeof(unsigned long long) * 3 * kPhases));
+    return MONSEL_OK;
+  });
+}
+#endif
 
 }  /* extern "C" */

@@ -22,6 +22,7 @@
 
 #include "../../include/monsel_gpu.h"
 #include "device_scene.h"
+#include "phase_timing.h"
 
 namespace monsel_dev {
 
@@ -87,6 +88,8 @@ struct Elec {
   uint32_t job;
   uint32_t nchild;
   uint32_t steps;
+  float safe;   /* lower bound on the distance to the nearest boundary findEndOfStep
+                 * would test from here (own shape and children); 0 = unknown */
 };
 
 MD float u01(uint32_t x) { return (float)(x >> 9) * 1.1920928955078125e-07f + 5.9604644775390625e-08f; }
@@ -186,14 +189,14 @@ MD int u_index0(float reduced, int len) {
 /* ------------------------------------------------- elastic (per element) */
 
 /* BrowningEmpiricalCrossSection.cu:24-29 (nm^2) */
-MD float browning_sigma(const DElem& el, float EeV) {
+MD float browning_sigma(const MONSEL_CS DElem& el, float EeV) {
   float e = EeV * 1.0e-3f;
   float re = __builtin_sqrtf(e);
   return 3.0e-4f * el.zp17 / (e + 0.005f * el.zp17 * re + 0.0007f * el.zp2 / re);
 }
 
 /* BrowningEmpiricalCrossSection.cu:31-45 -> cos(theta) */
-MD float browning_cos(const DElem& el, float EeV, float r1, float r2) {
+MD float browning_cos(const MONSEL_CS DElem& el, float EeV, float r1, float r2) {
   float e = EeV * 1.0e-3f;
   float r = (300.0f * e / el.z) + (el.zp3 / (3.0e5f * e));
   if (r1 <= r / (r + 1.f)) {
@@ -204,13 +207,13 @@ MD float browning_cos(const DElem& el, float EeV, float r1, float r2) {
 }
 
 /* ScreenedRutherfordScatteringAngle.cu:24-39 (sigma nm^2, cos theta) */
-MD float rutherford_sigma(const DElem& el, float EeV) {
+MD float rutherford_sigma(const MONSEL_CS DElem& el, float EeV) {
   const float K1 = 7.670843088080456e-38f / 1.60217653e-19f * 1.0e18f;
   const float K2 = 5.44967975966321e-19f / 1.60217653e-19f;
   float zp = el.rz13;
   return K1 * zp * (1.f + el.z) / (EeV + K2 * zp * zp);
 }
-MD float rutherford_cos(const DElem& el, float EeV, float r) {
+MD float rutherford_cos(const MONSEL_CS DElem& el, float EeV, float r) {
   const float K = 5.44968e-19f / 1.60217653e-19f;
   float alpha = K * el.rz23 / EeV;
   return 1.f - 2.0f * alpha * r / (1.f + alpha - r);
@@ -224,7 +227,7 @@ MD float mott_table_sigma(const DevScene& S, int mott, float EeV) {
   return kA0nm2 * lagrange4(red - (float)i0, f[0], f[1], f[2], f[3]);
 }
 
-MD float elem_sigma(const DevScene& S, const DElem& el, float EeV) {
+MD float elem_sigma(const DevScene& S, const MONSEL_CS DElem& el, float EeV) {
   if (el.kind == MONSEL_MECH_BROWNING_ELASTIC) return browning_sigma(el, EeV);
   if (el.kind == MONSEL_MECH_RUTHERFORD_ELASTIC) return rutherford_sigma(el, EeV);
   if (EeV < el.extrap) {
@@ -274,7 +277,7 @@ MD float nieminen_factor(float eps) {
  * `pending` marks a second half at depth d.  Below breakE the Nieminen form
  * kE (1 + 1.5 gamma len kE^1.5)^(-2/3) - kE; above it the Joy-Luo loss,
  * halved while it exceeds 10% of the energy. */
-MD float csd_compute(const DCSD& c, float len, float kE) {
+MD float csd_compute(const MONSEL_CS DCSD& c, float len, float kE) {
   if (!c.active || c.nce == 0) return 0.f;
   float E = kE, total = 0.f;
   uint32_t pending = 0;
@@ -389,7 +392,7 @@ MD T nu_poly(const T* xs, const T* f, T x) {
 
 /* NULagrangeInterpolation::d1 (:144-158). */
 template <typename T, int ORDER>
-MD T nu_d1(const DevScene& S, const DNUTable& t, T x0) {
+MD T nu_d1(const DevScene& S, const MONSEL_CS DNUTable& t, T x0) {
   const T* d = nu_base<T>(S);
   const T* xa = d + t.xoff[0];
   int i0 = nu_locate<T>(xa, t.n[0], ORDER, x0);
@@ -398,7 +401,7 @@ MD T nu_d1(const DevScene& S, const DNUTable& t, T x0) {
 
 /* NULagrangeInterpolation::d2 (:169-192); values are an [na][nb] block. */
 template <typename T, int ORDER>
-MD T nu_d2(const DevScene& S, const DNUTable& t, T x0, T x1) {
+MD T nu_d2(const DevScene& S, const MONSEL_CS DNUTable& t, T x0, T x1) {
   const T* d = nu_base<T>(S);
   const T* xa = d + t.xoff[0];
   const T* xb = d + t.xoff[1];
@@ -417,7 +420,7 @@ MD T nu_d2(const DevScene& S, const DNUTable& t, T x0, T x1) {
  * (outer abscissae taken from the start of the grid).  voff overrides the
  * value offset (the 4-D lookup walks 3-D slices). */
 template <typename T, int ORDER>
-MD T nu_d3(const DevScene& S, const DNUTable& t, int ax, int voff, T x0, T x1, T x2, bool compat_d3) {
+MD T nu_d3(const DevScene& S, const MONSEL_CS DNUTable& t, int ax, int voff, T x0, T x1, T x2, bool compat_d3) {
   const T* d = nu_base<T>(S);
   const T* xa = d + t.xoff[ax];
   const T* xb = d + t.xoff[ax + 1];
@@ -451,7 +454,7 @@ MD T nu_d3(const DevScene& S, const DNUTable& t, int ax, int voff, T x0, T x1, T
 
 /* NULagrangeInterpolation::d4 (:259-306). */
 template <typename T, int ORDER>
-MD T nu_d4(const DevScene& S, const DNUTable& t, const T* x, bool compat_d3) {
+MD T nu_d4(const DevScene& S, const MONSEL_CS DNUTable& t, const T* x, bool compat_d3) {
   const T* xa = nu_base<T>(S) + t.xoff[0];
   int i0 = nu_locate<T>(xa, t.n[0], ORDER, x[0]);
   T y[ORDER + 1];
@@ -463,7 +466,7 @@ MD T nu_d4(const DevScene& S, const DNUTable& t, const T* x, bool compat_d3) {
 
 template <typename T, int ORDER>
 MD T nu_interp_o(const DevScene& S, int table, const T* x, bool compat_d3) {
-  const DNUTable& t = S.nu[table];
+  const MONSEL_CS DNUTable& t = S.nu[table];
   switch (t.dim) {
     case 1: return nu_d1<T, ORDER>(S, t, x[0]);
     case 2: return nu_d2<T, ORDER>(S, t, x[0], x[1]);
@@ -490,17 +493,17 @@ MD double nu_interp(const DevScene& S, int table, int order, const double* x, bo
 
 /* ------------------------------------------------------ mechanisms ----- */
 
-MD float elastic_rate(const DevScene& S, const DMech& m, float EeV) {
+MD float elastic_rate(const DevScene& S, const MONSEL_CS DMech& m, float EeV) {
   float tot = 0.f;
   for (int i = 0; i < m.nelem; ++i) {
-    const DElem& el = S.elems[m.elem0 + i];
+    const MONSEL_CS DElem& el = S.elems[m.elem0 + i];
     tot += elem_sigma(S, el, EeV) * el.natoms;
   }
   return tot;
 }
 
 /* ScatterMechanism::scatterRate (1/nm) */
-MD float mech_rate(const DevScene& S, const DMech& m, float EeV) {
+MD float mech_rate(const DevScene& S, const MONSEL_CS DMech& m, float EeV) {
   switch (m.kind) {
     case MONSEL_MECH_NISTMOTT_ELASTIC:
     case MONSEL_MECH_BROWNING_ELASTIC:
@@ -572,7 +575,7 @@ MD float e0_from_dispersion(float Eq, float deltaE) {
 }
 
 /* TabulatedInelasticSM::pickBE (:153-207), default branching ratios; J. */
-MD double pick_be(const DMech& m, double Eq, double deltaE) {
+MD double pick_be(const MONSEL_CS DMech& m, double Eq, double deltaE) {
   const int ncore = (int)m.q[TP_NCORE];
   if (ncore > 0 && deltaE <= m.q[TP_CORE0]) return 0.;
   const double eIon = m.q[TP_E0DISP] != 0. ? (double)e0_from_dispersion((float)Eq, (float)deltaE) : deltaE;
@@ -595,7 +598,7 @@ MD void isotropic_dir(float u1, float u2, float& dx, float& dy, float& dz) {
 /* TabulatedInelasticSM::scatter (TabulatedInelasticSM.cu:209-434) in FP64,
  * in the reference's SI units (J); only the final rotations are FP32.  The
  * parameters come from DMech::q (J). */
-MD Spawn tabulated_scatter(const DevScene& S, const DMech& m, Elec& e, const float* u) {
+MD Spawn tabulated_scatter(const DevScene& S, const MONSEL_CS DMech& m, Elec& e, const float* u) {
   Spawn se;
   se.has = false;
   const bool cd3 = (S.compat & MONSEL_COMPAT_D3_ABSCISSAE) != 0;
@@ -711,7 +714,11 @@ MD Spawn tabulated_scatter(const DevScene& S, const DMech& m, Elec& e, const flo
 
 /* ScatterMechanism::scatter.  `rate` is this mechanism's scatterRate at
  * prevE, as summed when the step's free path was drawn. */
-MD Spawn mech_scatter(const DevScene& S, const DMech& m, Elec& e, float prevE, float rate, bool& trapped,
+/* TAB = false leaves the tabulated mechanism out of the code (its FP64
+ * scatter sets the register budget of every kernel that contains it); only
+ * for scenes without one. */
+template <bool TAB = true>
+MD Spawn mech_scatter(const DevScene& S, const MONSEL_CS DMech& m, Elec& e, float prevE, float rate, bool& trapped,
                       const float* u) {
   Spawn se;
   se.has = false;
@@ -719,6 +726,7 @@ MD Spawn mech_scatter(const DevScene& S, const DMech& m, Elec& e, float prevE, f
     case MONSEL_MECH_NISTMOTT_ELASTIC:
     case MONSEL_MECH_BROWNING_ELASTIC:
     case MONSEL_MECH_RUTHERFORD_ELASTIC: { /* SelectableElasticSM.cu:49-67 */
+      PT_BEGIN(t_el);
       /* first element whose running sum of sigma * atoms reaches u * rate;
        * the sum is re-formed in the order elastic_rate formed it */
       const float r = u[2] * rate;
@@ -726,10 +734,10 @@ MD Spawn mech_scatter(const DevScene& S, const DMech& m, Elec& e, float prevE, f
       float cum = elem_sigma(S, S.elems[m.elem0], prevE) * S.elems[m.elem0].natoms;
       while (idx < m.nelem - 1 && cum < r) {
         ++idx;
-        const DElem& el = S.elems[m.elem0 + idx];
+        const MONSEL_CS DElem& el = S.elems[m.elem0 + idx];
         cum += elem_sigma(S, el, prevE) * el.natoms;
       }
-      const DElem& el = S.elems[m.elem0 + idx];
+      const MONSEL_CS DElem& el = S.elems[m.elem0 + idx];
       /* the polar angle takes one uniform (two for Browning), the azimuth
        * the next one (SelectableElasticSM.cu:62-64) */
       float ca, ub;
@@ -749,23 +757,33 @@ MD Spawn mech_scatter(const DevScene& S, const DMech& m, Elec& e, float prevE, f
       float sb, cb;
       __sincosf(kTwoPI * ub, &sb, &cb);
       rotate_dir(e.dx, e.dy, e.dz, ca, sa, cb, sb);
+      PT_END(PH_ELASTIC, t_el);
       return se;
     }
     case MONSEL_MECH_FITTED_INELASTIC: { /* FittedInelSM.cu:13-18 */
+      PT_BEGIN(t_fi);
       float u_phi = u[2];
       float u_th = u[3];
       isotropic_dir(u_th, u_phi, se.dx, se.dy, se.dz);
       se.E = m.p[1];
       se.has = true;
+      PT_END(PH_FITTED, t_fi);
       return se;
     }
-    case MONSEL_MECH_POLARON_TRAP: /* GanachaudMokraniPolaronTrapSM.cu:15-20 */
+    case MONSEL_MECH_POLARON_TRAP: { /* GanachaudMokraniPolaronTrapSM.cu:15-20 */
+      PT_BEGIN(t_po);
       e.E = 0.f;
       trapped = true;
+      PT_END(PH_POLARON, t_po);
       return se;
+    }
     case MONSEL_MECH_PHONON: { /* GanachaudMokraniPhononInelasticSM.cu:19-44 */
+      PT_BEGIN(t_ph);
       float kE0 = e.E;
-      if (kE0 < m.p[0]) return se;
+      if (kE0 < m.p[0]) {
+        PT_END(PH_PHONON, t_ph);
+        return se;
+      }
       float x = m.p[0] / kE0;
       const float r0 = u[2], r1 = u[3];
       float ct;
@@ -781,10 +799,17 @@ MD Spawn mech_scatter(const DevScene& S, const DMech& m, Elec& e, float prevE, f
       __sincosf(kTwoPI * r1, &sb, &cb);
       rotate_dir(e.dx, e.dy, e.dz, ct, st, cb, sb);
       e.E = kE0 - m.p[0];
+      PT_END(PH_PHONON, t_ph);
       return se;
     }
     case MONSEL_MECH_TABULATED_INELASTIC:
-      return tabulated_scatter(S, m, e, u);
+      if constexpr (TAB) {
+        PT_BEGIN(t_tb);
+        Spawn r = tabulated_scatter(S, m, e, u);
+        PT_END(PH_TABULATED, t_tb);
+        return r;
+      }
+      break;
   }
   return se;
 }
@@ -807,7 +832,7 @@ MD bool plane_tie_break(float nx, float ny, float nz) {
 }
 
 /* Parameter interval of the line p + t d inside a convex primitive. */
-MD Interval prim_interval(const DevScene& S, const DPrim& pr, float px, float py, float pz, float dx, float dy,
+MD Interval prim_interval(const DevScene& S, const MONSEL_CS DPrim& pr, float px, float py, float pz, float dx, float dy,
                           float dz) {
   Interval iv;
   iv.tin = -kInf;
@@ -904,7 +929,7 @@ MD Interval prim_interval(const DevScene& S, const DPrim& pr, float px, float py
 }
 
 /* Outward normal of primitive face `f` at point q. */
-MD void prim_normal(const DevScene& S, const DPrim& pr, int f, float qx, float qy, float qz, float& nx, float& ny,
+MD void prim_normal(const DevScene& S, const MONSEL_CS DPrim& pr, int f, float qx, float qy, float qz, float& nx, float& ny,
                     float& nz) {
   if (pr.kind == PRIM_PLANES) {
     float4 pl = S.planes[pr.first_plane + (f < 0 ? 0 : f)];
@@ -934,7 +959,7 @@ MD void prim_normal(const DevScene& S, const DPrim& pr, int f, float qx, float q
   nz = wz * inv;
 }
 
-MD bool prim_contains(const DevScene& S, const DPrim& pr, float px, float py, float pz) {
+MD bool prim_contains(const DevScene& S, const MONSEL_CS DPrim& pr, float px, float py, float pz) {
   if (pr.kind == PRIM_PLANES) {
     /* MultiPlaneShape.cu:23-27 for every plane */
     for (int k = 0; k < pr.n_planes; ++k) {
@@ -953,10 +978,34 @@ MD bool prim_contains(const DevScene& S, const DPrim& pr, float px, float py, fl
   return wx * wx + wy * wy + wz * wz <= pr.r2;
 }
 
-MD bool truth(const DShape& sh, uint32_t bits) { return (sh.truth[bits >> 5] >> (bits & 31u)) & 1u; }
+/* Lower bound on the distance from p to the surface of a convex primitive:
+ * |max_k g_k| over its constraints, g_k the signed distance to constraint k
+ * (plane rows scaled by sdscale <= 1/|n|).  Inside, -max g_k is the distance
+ * to the nearest face; outside, max g_k is at most the distance to the
+ * primitive. */
+MD float prim_safety(const DevScene& S, const MONSEL_CS DPrim& pr, float px, float py, float pz) {
+  if (pr.kind == PRIM_PLANES) {
+    float g = -kInf;
+    for (int k = 0; k < pr.n_planes; ++k) {
+      const float4 pl = S.planes[pr.first_plane + k];
+      g = fmaxf(g, pl.x * px + pl.y * py + pl.z * pz - pl.w);
+    }
+    return fabsf(g) * pr.sdscale;
+  }
+  const float wx = px - pr.e0[0], wy = py - pr.e0[1], wz = pz - pr.e0[2];
+  if (pr.kind == PRIM_CYLINDER) {
+    const float apar = wx * pr.ax[0] + wy * pr.ax[1] + wz * pr.ax[2];
+    const float qx = wx - apar * pr.ax[0], qy = wy - apar * pr.ax[1], qz = wz - apar * pr.ax[2];
+    const float radial = __builtin_sqrtf(qx * qx + qy * qy + qz * qz) - __builtin_sqrtf(pr.r2);
+    return fabsf(fmaxf(radial, fmaxf(-apar, apar - pr.len)));
+  }
+  return fabsf(__builtin_sqrtf(wx * wx + wy * wy + wz * wz) - __builtin_sqrtf(pr.r2));
+}
+
+MD bool truth(const MONSEL_CS DShape& sh, uint32_t bits) { return (sh.truth[bits >> 5] >> (bits & 31u)) & 1u; }
 
 MD bool shape_contains(const DevScene& S, int s, float px, float py, float pz) {
-  const DShape& sh = S.shapes[s];
+  const MONSEL_CS DShape& sh = S.shapes[s];
   uint32_t bits = 0;
   for (int k = 0; k < sh.n_prims; ++k)
     if (prim_contains(S, S.prims[sh.first_prim + k], px, py, pz)) bits |= 1u << k;
@@ -980,7 +1029,7 @@ MD ShapeHit shape_first(const DevScene& S, int s, float px, float py, float pz, 
   __shared__ int8_t l_fin[kMaxPrims][kLaneBlock];
   __shared__ int8_t l_fout[kMaxPrims][kLaneBlock];
   const int tid = threadIdx.x;
-  const DShape& sh = S.shapes[s];
+  const MONSEL_CS DShape& sh = S.shapes[s];
   ShapeHit h;
   h.t = kInf;
   h.nx = h.ny = h.nz = 0.f;
@@ -1051,7 +1100,7 @@ MD ShapeHit shape_first(const DevScene& S, int s, float px, float py, float pz, 
 MD int containing_sub_region(const DevScene& S, int r, float px, float py, float pz) {
   if (!shape_contains(S, S.regions[r].shape, px, py, pz)) return -1;
   for (int guard = 0; guard < 64; ++guard) {
-    const DRegion& rg = S.regions[r];
+    const MONSEL_CS DRegion& rg = S.regions[r];
     int next = -1;
     for (int c = 0; c < rg.n_children; ++c) {
       int ch = S.children[rg.first_child + c];
@@ -1075,6 +1124,19 @@ MD bool is_containing_region(const DevScene& S, int r, int target) {
   return false;
 }
 
+/* Lower bound on the distance from p to every primitive surface of region
+ * r's shape and of its children's shapes -- the surfaces findEndOfStep tests
+ * from r.  A step from p shorter than this cannot end at a boundary. */
+MD float region_safety(const DevScene& S, int r, float px, float py, float pz) {
+  const MONSEL_CS DRegion& rg = S.regions[r];
+  float d = kInf;
+  for (int c = -1; c < rg.n_children; ++c) {
+    const MONSEL_CS DShape& sh = S.shapes[S.regions[c < 0 ? r : S.children[rg.first_child + c]].shape];
+    for (int k = 0; k < sh.n_prims; ++k) d = fminf(d, prim_safety(S, S.prims[sh.first_prim + k], px, py, pz));
+  }
+  return d;
+}
+
 /* Result of RegionBase::findEndOfStep. */
 struct StepEnd {
   float t;        /* distance moved (<= s) */
@@ -1083,6 +1145,8 @@ struct StepEnd {
   float ownNx, ownNy, ownNz;   /* current shape's crossing normal (this step) */
   int winner;                  /* child region whose shape won, -1 = own shape */
   float winNx, winNy, winNz;
+  float safe;   /* no boundary: lower bound on the distance from the end point to any surface
+                 * findEndOfStep tests from this region (region_safety); else 0 */
 };
 
 /* Displacement that carries a point across a boundary at coordinate scale |p|. */
@@ -1096,7 +1160,7 @@ MD float nudge_nm(float px, float py, float pz) {
 MD StepEnd find_end_of_step(const DevScene& S, int r, float px, float py, float pz, float dx, float dy, float dz,
                             float s) {
   StepEnd se;
-  const DRegion& rg = S.regions[r];
+  const MONSEL_CS DRegion& rg = S.regions[r];
   float t = kInf;
   int base = r;
   se.winner = -1;
@@ -1123,6 +1187,7 @@ MD StepEnd find_end_of_step(const DevScene& S, int r, float px, float py, float 
   if (t <= s) {
     se.t = t;
     se.boundary = true;
+    se.safe = 0.f;
     if (rg.parent < 0 && se.winner < 0) {
       se.next = -1; /* through the chamber wall */
       return se;
@@ -1142,6 +1207,7 @@ MD StepEnd find_end_of_step(const DevScene& S, int r, float px, float py, float 
   se.t = s;
   se.boundary = false;
   se.next = r;
+  se.safe = region_safety(S, r, px + s * dx, py + s * dy, pz + s * dz);
   return se;
 }
 

@@ -1,0 +1,65 @@
+"""Per-phase wave timing of the transport kernel on the line-scan workload.
+
+Needs a library built with `python -m cudamonsel_amd.build --phase-timing`
+(csrc/phase_timing.h).  Runs the 0.5 keV PMMA-line-on-Si scan (128 pixels at
+1 nm pitch, primaries per pixel from the command line), reads the kernel's
+counters and prints, per phase, its share of the wave cycles of the whole
+loop, the cycles per entry and the lane utilisation on entry.
+
+Usage: python -m cudamonsel_amd.phase_timing [per_pixel] [repeats]
+"""
+import ctypes
+import sys
+
+import numpy as np
+
+PHASES = ["claim", "contain", "rates", "geom", "csd", "select", "elastic", "fitted", "polaron", "phonon",
+          "tabulated", "barrier", "step", "tail", "rates_uni", "geom_uni"]
+
+
+def read_counters(lib, device=0):
+    fn = getattr(lib, "monsel_phase_timing_read", None)
+    if fn is None:
+        raise RuntimeError("libmonsel_gpu.so has no phase counters: build it with "
+                           "`python -m cudamonsel_amd.build --phase-timing`")
+    out = (ctypes.c_uint64 * (3 * len(PHASES)))()
+    if fn(ctypes.c_int(device), out, ctypes.c_int(3 * len(PHASES))) != 0:
+        raise RuntimeError("monsel_phase_timing_read failed")
+    return np.array(out[:], dtype=np.float64).reshape(len(PHASES), 3)
+
+
+def table(c):
+    """Rows (phase, share of loop cycles, cycles per entry, entries, lane utilisation)."""
+    loop = c[PHASES.index("claim"), 0] + c[PHASES.index("step"), 0] + c[PHASES.index("tail"), 0]
+    rows = []
+    for k, name in enumerate(PHASES):
+        cyc, ent, lanes = c[k]
+        rows.append((name, cyc / loop if loop else 0.0, cyc / ent if ent else 0.0, int(ent),
+                     lanes / (64.0 * ent) if ent else 0.0))
+    return rows
+
+
+def main():
+    per_pixel = int(sys.argv[1]) if len(sys.argv) > 1 else 2000
+    repeats = int(sys.argv[2]) if len(sys.argv) > 2 else 2
+    import cudamonsel_amd as M
+    from cudamonsel_amd import scenes
+    sc = scenes.line_on_substrate(h_nm=50.0, w_nm=20.0, theta_deg=3.0, r_nm=5.0, length_nm=1000.0,
+                                  line="PMMA", substrate="Si", beam_eV=500.0, sigma_nm=0.5, compat=True)
+    centers = scenes.scan_centers(np.arange(-64.0, 64.0, 1.0), [0.0], sc.beam_z)
+    lib = M.gpu_lib()
+    sc.monte.runPixels(centers, per_pixel, 50, 50.0)   # warm-up
+    read_counters(lib)
+    steps = 0
+    for _ in range(repeats):
+        r = sc.monte.runPixels(centers, per_pixel, 50, 50.0)
+        steps += r.stats.steps
+    c = read_counters(lib)
+    print(f"line scan 128 px x {per_pixel} x {repeats}: {steps} electron steps")
+    print(f"{'phase':<10} {'share':>7} {'cyc/entry':>10} {'entries':>12} {'lanes':>6}")
+    for name, share, cpe, ent, util in table(c):
+        print(f"{name:<10} {share:7.3f} {cpe:10.0f} {ent:12d} {util:6.3f}")
+
+
+if __name__ == "__main__":
+    main()

@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define MONSEL_ABI_VERSION 2u
+#define MONSEL_ABI_VERSION 3u
 
 typedef enum monsel_status {
   MONSEL_OK = 0,
@@ -276,6 +276,8 @@ typedef struct monsel_tuning {
   int32_t steps_per_launch;     /* regroup interval: steps an electron takes per launch (0 = unlimited) */
   int32_t admission_order;      /* new primaries go first to: 0 the oldest job, 1 the job with fewest left */
   int32_t sort_queue;           /* 0: sort each launch's queue by region and energy class; -1: keep spawn order */
+  int32_t geometry_batch;       /* lanes of a wave whose step may reach a boundary that wait before the wave
+                                 * runs findEndOfStep for them (0 = default 1 = no waiting, <= 64) */
 } monsel_tuning;
 
 typedef struct monsel_ctx_s* monsel_ctx;
