@@ -56,7 +56,7 @@ SYMBOLS = [
     "monsel_job_submit", "monsel_job_poll", "monsel_job_wait", "monsel_job_read", "monsel_job_stats_get",
     "monsel_job_release", "monsel_run", "monsel_probe_rates", "monsel_probe_elastic_angle", "monsel_probe_csd",
     "monsel_probe_find_end_of_step", "monsel_probe_locate", "monsel_probe_nu_table", "monsel_probe_scatter",
-    "monsel_probe_barrier",
+    "monsel_probe_barrier", "monsel_probe_safety",
 ]
 
 
@@ -108,6 +108,13 @@ class Probe:
         _check(self.L.monsel_probe_find_end_of_step(self.ctx, self.scene, int(region), n, _fp(a), _fp(b), _fp(u),
                                                     nxt.ctypes.data_as(c_int_p), _fp(nrm)), "probe_find_end_of_step")
         return u, nxt, nrm
+
+    def safety(self, region, pos_nm):
+        p = np.ascontiguousarray(pos_nm, dtype=np.float32).reshape(-1, 3)
+        out = np.zeros(p.shape[0], np.float32)
+        _check(self.L.monsel_probe_safety(self.ctx, self.scene, int(region), p.shape[0], _fp(p), _fp(out)),
+               "probe_safety")
+        return out
 
     def locate(self, pos_nm):
         p = np.ascontiguousarray(pos_nm, dtype=np.float32).reshape(-1, 3)

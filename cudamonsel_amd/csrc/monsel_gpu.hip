@@ -1086,6 +1086,12 @@ __global__ void k_probe_step_end(DevScene S, int region, int n, const float* p0,
   out_n[3 * i + 2] = se.boundary ? se.winNz : 0.f;
 }
 
+__global__ void k_probe_safety(DevScene S, int region, int n, const float* p, float* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = region_safety(S, region, p[3 * i], p[3 * i + 1], p[3 * i + 2]);
+}
+
 __global__ void k_probe_locate(DevScene S, int n, const float* p, int* out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -1489,8 +1495,8 @@ HostScene compile_scene(const monsel_scene_desc* d) {
           H.planes.push_back(make_float4((float)n[0], (float)n[1], (float)n[2], (float)b));
           maxn = std::max(maxn, sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]));
         }
-        /* 1/|n| rounded down a little so that FP32 distance bounds stay below the true distance */
-        p.sdscale = maxn > 0. ? (float)(0.999 / maxn) : 0.f;
+        /* min over the planes of 1/|n|, rounded down (distance bounds) */
+        p.sdscale = maxn > 0. ? std::nextafter((float)(1.0 / maxn), 0.f) : 0.f;
       } else if (sh.kind == MONSEL_SHAPE_CYLINDER) {
         p.kind = PRIM_CYLINDER;
         p.sdscale = 1.f;
@@ -1526,6 +1532,20 @@ HostScene compile_scene(const monsel_scene_desc* d) {
     }
     for (uint32_t bits = 0; bits < (1u << ds.n_prims); ++bits)
       if (eval_tree(d, s, leaves, bits)) ds.truth[bits >> 5] |= 1u << (bits & 31u);
+    if (ds.n_prims <= kMaxCstPrims) {
+      const uint32_t nb = 1u << ds.n_prims;
+      auto tv = [&](uint32_t b) { return (ds.truth[b >> 5] >> (b & 31u)) & 1u; };
+      for (uint32_t F = 0; F < nb; ++F)
+        for (uint32_t B = 0; B < nb; ++B) {
+          bool same = true;
+          for (uint32_t S = F;; S = (S - 1) & F) { /* every subset of F */
+            if (tv(B ^ S) != tv(B)) same = false;
+            if (S == 0) break;
+          }
+          if (same) ds.cst[F] |= 1u << B;
+        }
+      ds.has_cst = 1;
+    }
     shapeIdx[s] = (int)H.shapes.size();
     H.shapes.push_back(ds);
   }
@@ -2585,6 +2605,23 @@ int monsel_probe_find_end_of_step(monsel_ctx ctx, monsel_scene_id scene, int reg
     HIP_TRY(hipMemcpy(out_u, dU.d, 4 * (size_t)n, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(out_next, dX.d, 4 * (size_t)n, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(out_normal, dN.d, 12 * (size_t)n, hipMemcpyDeviceToHost));
+    return MONSEL_OK;
+  });
+}
+
+int monsel_probe_safety(monsel_ctx ctx, monsel_scene_id scene, int region, int n, const float* pos, float* out) {
+  std::shared_ptr<SceneDev> sd;
+  int st = probe_prologue(ctx, scene, sd);
+  if (st) return st;
+  if (region < 0 || region >= sd->dev.n_regions || n < 0 || (n > 0 && (!pos || !out)))
+    return set_err(MONSEL_E_INVALID, "probe_safety arguments");
+  if (n == 0) return MONSEL_OK;
+  return guarded([&] {
+    Scratch<float> dP(3 * (size_t)n), dO(n);
+    HIP_TRY(hipMemcpy(dP.d, pos, 12 * (size_t)n, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_probe_safety, dim3(grid_for(n)), dim3(256), 0, 0, sd->dev, region, n, dP.d, dO.d);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(out, dO.d, 4 * (size_t)n, hipMemcpyDeviceToHost));
     return MONSEL_OK;
   });
 }

@@ -978,28 +978,28 @@ MD bool prim_contains(const DevScene& S, const MONSEL_CS DPrim& pr, float px, fl
   return wx * wx + wy * wy + wz * wz <= pr.r2;
 }
 
-/* Lower bound on the distance from p to the surface of a convex primitive:
- * |max_k g_k| over its constraints, g_k the signed distance to constraint k
- * (plane rows scaled by sdscale <= 1/|n|).  Inside, -max g_k is the distance
- * to the nearest face; outside, max g_k is at most the distance to the
- * primitive. */
-MD float prim_safety(const DevScene& S, const MONSEL_CS DPrim& pr, float px, float py, float pz) {
+/* max_k g_k over a convex primitive's constraints, g_k the signed distance
+ * to constraint k (plane rows scaled by sdscale <= 1/|n|): <= 0 inside.
+ * Its magnitude bounds the distance from p to the primitive's surface from
+ * below -- inside, -max g_k is the distance to the nearest face; outside,
+ * max g_k is at most the distance to the primitive. */
+MD float prim_gmax(const DevScene& S, const MONSEL_CS DPrim& pr, float px, float py, float pz) {
   if (pr.kind == PRIM_PLANES) {
     float g = -kInf;
     for (int k = 0; k < pr.n_planes; ++k) {
       const float4 pl = S.planes[pr.first_plane + k];
       g = fmaxf(g, pl.x * px + pl.y * py + pl.z * pz - pl.w);
     }
-    return fabsf(g) * pr.sdscale;
+    return g * pr.sdscale;
   }
   const float wx = px - pr.e0[0], wy = py - pr.e0[1], wz = pz - pr.e0[2];
   if (pr.kind == PRIM_CYLINDER) {
     const float apar = wx * pr.ax[0] + wy * pr.ax[1] + wz * pr.ax[2];
     const float qx = wx - apar * pr.ax[0], qy = wy - apar * pr.ax[1], qz = wz - apar * pr.ax[2];
     const float radial = __builtin_sqrtf(qx * qx + qy * qy + qz * qz) - __builtin_sqrtf(pr.r2);
-    return fabsf(fmaxf(radial, fmaxf(-apar, apar - pr.len)));
+    return fmaxf(radial, fmaxf(-apar, apar - pr.len));
   }
-  return fabsf(__builtin_sqrtf(wx * wx + wy * wy + wz * wz) - __builtin_sqrtf(pr.r2));
+  return __builtin_sqrtf(wx * wx + wy * wy + wz * wz) - __builtin_sqrtf(pr.r2);
 }
 
 MD bool truth(const MONSEL_CS DShape& sh, uint32_t bits) { return (sh.truth[bits >> 5] >> (bits & 31u)) & 1u; }
@@ -1124,16 +1124,57 @@ MD bool is_containing_region(const DevScene& S, int r, int target) {
   return false;
 }
 
-/* Lower bound on the distance from p to every primitive surface of region
- * r's shape and of its children's shapes -- the surfaces findEndOfStep tests
- * from r.  A step from p shorter than this cannot end at a boundary. */
+/* Lower bound on the distance from p to the surface of shape sh.  Moving
+ * less than the distance D_k to primitive k's surface cannot change k's
+ * inside bit; taking the primitives in increasing D_k, the bound is the
+ * first D_k at which the bits that may have changed can change the shape's
+ * truth value (cst table), so surfaces inside the shape, or shared by
+ * primitives whose crossing does not matter, do not limit it.  Shapes
+ * without the table: the nearest primitive surface. */
+MD float shape_safety(const DevScene& S, const MONSEL_CS DShape& sh, float px, float py, float pz) {
+  const int n = sh.n_prims;
+  if (!sh.has_cst) {
+    float d = kInf;
+    for (int k = 0; k < n; ++k) d = fminf(d, fabsf(prim_gmax(S, S.prims[sh.first_prim + k], px, py, pz)));
+    return d;
+  }
+  float d[kMaxCstPrims];
+  uint32_t bits = 0;
+#pragma unroll
+  for (int k = 0; k < kMaxCstPrims; ++k) {
+    d[k] = kInf;
+    if (k < n) {
+      const float g = prim_gmax(S, S.prims[sh.first_prim + k], px, py, pz);
+      d[k] = fabsf(g);
+      if (g <= 0.f) bits |= 1u << k;
+    }
+  }
+  uint32_t F = 0;
+#pragma unroll
+  for (int m = 0; m < kMaxCstPrims; ++m) {
+    float dmin = kInf;
+    int kmin = -1;
+#pragma unroll
+    for (int k = 0; k < kMaxCstPrims; ++k)
+      if (!((F >> k) & 1u) && k < n && d[k] <= dmin) {
+        dmin = d[k];
+        kmin = k;
+      }
+    if (kmin < 0) break;
+    F |= 1u << kmin;
+    if (!((sh.cst[F] >> bits) & 1u)) return dmin;
+  }
+  return kInf;
+}
+
+/* Lower bound on the distance from p to the surfaces findEndOfStep tests
+ * from region r: r's shape and its children's shapes.  A step from p shorter
+ * than this cannot end at a boundary. */
 MD float region_safety(const DevScene& S, int r, float px, float py, float pz) {
   const MONSEL_CS DRegion& rg = S.regions[r];
   float d = kInf;
-  for (int c = -1; c < rg.n_children; ++c) {
-    const MONSEL_CS DShape& sh = S.shapes[S.regions[c < 0 ? r : S.children[rg.first_child + c]].shape];
-    for (int k = 0; k < sh.n_prims; ++k) d = fminf(d, prim_safety(S, S.prims[sh.first_prim + k], px, py, pz));
-  }
+  for (int c = -1; c < rg.n_children; ++c)
+    d = fminf(d, shape_safety(S, S.shapes[S.regions[c < 0 ? r : S.children[rg.first_child + c]].shape], px, py, pz));
   return d;
 }
 

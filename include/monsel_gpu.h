@@ -339,6 +339,13 @@ int monsel_probe_find_end_of_step(monsel_ctx ctx, monsel_scene_id scene, int reg
                                   const float* p0, const float* p1, float* out_u, int* out_next,
                                   float* out_normal);
 
+/* Lower bound (nm) on the distance from pos to every surface findEndOfStep
+ * tests from `region` (its shape and its children's shapes).  No reference
+ * counterpart: MonteCarloSS::takeStep (MonteCarloSS.cu:107-163) runs
+ * findEndOfStep on every step, and the transport kernel skips it for a step
+ * shorter than this bound. */
+int monsel_probe_safety(monsel_ctx ctx, monsel_scene_id scene, int region, int n, const float* pos, float* out);
+
 /* RegionBase::containingSubRegion from the chamber (RegionBase.cu:50-60). */
 int monsel_probe_locate(monsel_ctx ctx, monsel_scene_id scene, int n, const float* pos, int* out_region);
 

@@ -94,6 +94,58 @@ def test_find_end_of_step(oracle, compat):
     assert agree / tot >= 0.995
 
 
+@pytest.mark.parametrize("geometry", ["line-compat", "line-fixed", "three-lines", "holes", "step"])
+def test_safety_bounds_the_next_boundary(oracle, geometry):
+    """The transport kernel skips findEndOfStep for a step shorter than the
+    safety bound (monsel_probe_safety).  Pins the bound: no segment from a
+    point shorter than its bound reaches a boundary, on the device or in the
+    FP64 oracle; in the substrate away from every line the bound is the depth
+    (the one surface there); and the bound is not trivially small."""
+    from cudamonsel_amd import scenes
+    from cudamonsel_amd.abi import probe_for
+    sc = {"line-compat": lambda: scenes.line_on_substrate(compat=True),
+          "line-fixed": lambda: scenes.line_on_substrate(compat=False),
+          "three-lines": lambda: scenes.random_lines(seed=101),
+          "holes": lambda: scenes.contact_holes(radius_nm=12.0, depth_nm=60.0),
+          "step": lambda: scenes.step_edge("Si")}[geometry]()
+    pr, cs = probe_for(sc)
+    osc = oracle.OracleScene(cs)
+    rng = np.random.default_rng(7)
+    n = 20000
+    p0 = np.stack([rng.uniform(-90, 90, n), rng.uniform(-40, 40, n), rng.uniform(-100, 40, n)], 1)
+    p0 = p0.astype(np.float32).astype(np.float64)
+    reg = osc.locate(p0)
+    saf = np.full(n, np.nan)
+    for r in np.unique(reg):
+        if r >= 0:
+            saf[reg == r] = pr.safety(r, p0[reg == r])
+    ok = (reg >= 0) & np.isfinite(saf)
+    assert ok.mean() > 0.99
+    assert np.all(saf[ok] >= 0.0)
+    if geometry != "step":
+        # points below z = 0 beyond the lines and holes: the bound is the
+        # distance to the planes z = 0 (and z = 60 nm, the layer's bottom)
+        far = ok & (reg == reg[np.argmax(p0[:, 2])]) & (np.abs(p0[:, 0]) > 80) & (p0[:, 2] > 0.5)
+        z = p0[far, 2]
+        expect = np.minimum(z, 60.0 - z) if geometry == "holes" else z
+        assert far.sum() > 100 and np.allclose(saf[far], expect, rtol=1e-5, atol=1e-4)
+    k = 4
+    sel = ok & (saf > 2e-3) & (saf < 1e6)
+    P = np.repeat(p0[sel], k, 0)
+    R = np.repeat(reg[sel], k)
+    L = np.repeat(0.999 * saf[sel] - 1e-3, k)
+    d = rng.normal(size=(P.shape[0], 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    P1 = (P + d * L[:, None]).astype(np.float32).astype(np.float64)
+    for r in np.unique(R):
+        m = R == r
+        gu, gnx, _ = pr.find_end_of_step(r, P[m], P1[m])
+        ou, onx, _ = osc.find_end_of_step(r, P[m], P1[m])
+        assert np.all(gu > 1.0) and np.all(gnx == r), (geometry, r, int(np.sum(gu <= 1.0)))
+        assert np.all(ou > 1.0), (geometry, r, int(np.sum(ou <= 1.0)))
+    assert np.median(saf[ok]) > 1.0
+
+
 @pytest.mark.parametrize("scale0", [1.0, 1.602e-16], ids=["unit", "joule"])
 def test_nu_tables(oracle, tmp_path, M, scale0):
     """Tabulated-inelastic lookups; the joule case puts the first axis in the
