@@ -126,6 +126,8 @@ struct DMSM {
   int n_mech;
   int mech0;
   int barrier_kind;
+  int elastic_k;   /* first elastic mechanism (index within this model), -1 if none */
+  int pad[3];
   float minE;      /* eV, may be -inf */
   float cb;        /* eV, material conduction-band bottom */
   float lambdaFactor; /* ExpQMBarrierSM lambda factor, 1/sqrt(eV) */

@@ -305,6 +305,14 @@ __device__ __forceinline__ float* lane_rates() {
   return l_rate + threadIdx.x;
 }
 
+/* Per-lane terms sigma_i * atoms_i of the elastic mechanism's rate (the
+ * region model's first elastic mechanism), kept between the step's start and
+ * its scatter so that the element choice reuses them. */
+__device__ __forceinline__ float* lane_esig() {
+  __shared__ float l_esig[kMaxElem * kLaneBlock];
+  return l_esig + threadIdx.x;
+}
+
 /* What the start of a step leaves for its end. */
 struct StepStart {
   float freePath;
@@ -331,7 +339,21 @@ __device__ __forceinline__ void begin_step_in(const DevScene& S, int region, con
     freePath = 1.0e9f; /* NullMaterialScatterModel.cu:32-35: 1 m */
   } else {
     for (int k = 0; k < msm.n_mech; ++k) {
-      const float rk = mech_rate(S, S.mechs[msm.mech0 + k], e.E);
+      const MONSEL_CS DMech& m = S.mechs[msm.mech0 + k];
+      float rk;
+      if (k == msm.elastic_k) {
+        /* elastic_rate, keeping its terms */
+        float* esig = lane_esig();
+        rk = 0.f;
+        for (int i = 0; i < m.nelem; ++i) {
+          const MONSEL_CS DElem& el = S.elems[m.elem0 + i];
+          const float c = elem_sigma(S, el, e.E) * el.natoms;
+          esig[i * kLaneBlock] = c;
+          rk += c;
+        }
+      } else {
+        rk = mech_rate(S, m, e.E);
+      }
       rates[k * kLaneBlock] = rk;
       total += rk;
     }
@@ -449,7 +471,8 @@ __device__ __forceinline__ int finish_step_in(const DevScene& S, const JobRec* j
       u[3] = st.u3;
       step_uniforms_hi(e, kPhiloxKey0, kPhiloxKey1, u);
       PT_END(PH_SELECT, t_se);
-      sp = mech_scatter<TAB>(S, S.mechs[msm.mech0 + idx], e, prevE, rate, trapped, u);
+      sp = mech_scatter<TAB>(S, S.mechs[msm.mech0 + idx], e, prevE, rate, trapped, u,
+                             idx == msm.elastic_k ? lane_esig() : nullptr);
       if (sp.has) {
         /* trackSecondaryElectron (MonteCarloSS.cu:165-174) */
         if (!(sp.E > msm.minE)) sp.has = false;
@@ -1306,6 +1329,7 @@ HostScene compile_scene(const monsel_scene_desc* d) {
     m.n_mech = md.is_null ? 0 : md.n_mech;
     m.mech0 = (int)H.mechs.size();
     m.barrier_kind = md.barrier_kind;
+    m.elastic_k = -1;
     m.minE = (float)(md.min_e_for_tracking / kE);
     m.cb = (float)(mat.energy_cb_bottom / kE);
     m.lambda = (float)(md.barrier_lambda * 1e9);
@@ -1323,6 +1347,7 @@ HostScene compile_scene(const monsel_scene_desc* d) {
         case MONSEL_MECH_NISTMOTT_ELASTIC:
         case MONSEL_MECH_BROWNING_ELASTIC:
         case MONSEL_MECH_RUTHERFORD_ELASTIC: {
+          if (m.elastic_k < 0) m.elastic_k = k;
           dm.elem0 = (int)H.elems.size();
           dm.nelem = mat.n_elements;
           for (int j = 0; j < mat.n_elements; ++j) {

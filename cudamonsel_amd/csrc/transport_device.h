@@ -719,7 +719,7 @@ MD Spawn tabulated_scatter(const DevScene& S, const MONSEL_CS DMech& m, Elec& e,
  * for scenes without one. */
 template <bool TAB = true>
 MD Spawn mech_scatter(const DevScene& S, const MONSEL_CS DMech& m, Elec& e, float prevE, float rate, bool& trapped,
-                      const float* u) {
+                      const float* u, const float* esig = nullptr) {
   Spawn se;
   se.has = false;
   switch (m.kind) {
@@ -728,14 +728,16 @@ MD Spawn mech_scatter(const DevScene& S, const MONSEL_CS DMech& m, Elec& e, floa
     case MONSEL_MECH_RUTHERFORD_ELASTIC: { /* SelectableElasticSM.cu:49-67 */
       PT_BEGIN(t_el);
       /* first element whose running sum of sigma * atoms reaches u * rate;
-       * the sum is re-formed in the order elastic_rate formed it */
+       * the sum is re-formed in the order elastic_rate formed it, from the
+       * rate's own terms when the caller kept them (esig, lane stride
+       * kLaneBlock) */
       const float r = u[2] * rate;
       int idx = 0;
-      float cum = elem_sigma(S, S.elems[m.elem0], prevE) * S.elems[m.elem0].natoms;
+      float cum = esig ? esig[0] : elem_sigma(S, S.elems[m.elem0], prevE) * S.elems[m.elem0].natoms;
       while (idx < m.nelem - 1 && cum < r) {
         ++idx;
         const MONSEL_CS DElem& el = S.elems[m.elem0 + idx];
-        cum += elem_sigma(S, el, prevE) * el.natoms;
+        cum += esig ? esig[idx * kLaneBlock] : elem_sigma(S, el, prevE) * el.natoms;
       }
       const MONSEL_CS DElem& el = S.elems[m.elem0 + idx];
       /* the polar angle takes one uniform (two for Browning), the azimuth
