@@ -46,7 +46,7 @@ class Tuning(ctypes.Structure):
         ("queue_capacity", ctypes.c_int32), ("max_jobs_in_flight", ctypes.c_int32),
         ("primaries_per_launch", ctypes.c_int32), ("steps_per_launch", ctypes.c_int32),
         ("admission_order", ctypes.c_int32), ("sort_queue", ctypes.c_int32),
-        ("geometry_batch", ctypes.c_int32),
+        ("geometry_batch", ctypes.c_int32), ("spawn_handoff", ctypes.c_int32),
     ]
 
 

@@ -137,7 +137,14 @@ struct GenDev {
   unsigned int* esc_count;
   int steps_per_launch;      /* regroup interval: steps before an electron returns to the queue (0 = none) */
   int geo_batch;             /* lanes of a wave that wait for findEndOfStep before the wave runs it */
+  int handoff;               /* new secondaries go to idle lanes of their wave first */
 };
+
+/* per-wave LDS table of the handoff: lane of the k-th spawning lane */
+__device__ __forceinline__ int* lane_map() {
+  __shared__ int l_giver[(kLaneBlock / 64) * 64];
+  return l_giver + (threadIdx.x & ~63u);
+}
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
@@ -725,15 +732,18 @@ k_transport(DevScene S, GenDev G) {
       PT_END(PH_STEP, t_st);
     }
     PT_BEGIN(t_ta);
-    /* secondaries to the next launch */
+    /* a new secondary starts where its parent's step ended, in the region
+     * the parent is in (trackSecondaryElectron, MonteCarloSS.cu:165-174);
+     * its state is set aside before any lane takes a new electron below */
     uint32_t clo = 0, chi = 0;
     if (sp.has) {
       const uint64_t lin = child_lineage(((uint64_t)e.lin_hi << 32) | e.lin_lo, e.nchild++);
       clo = (uint32_t)lin;
       chi = (uint32_t)(lin >> 32);
     }
-    queue_append(G, sp.has, e.px, e.py, e.pz, sp.dx, sp.dy, sp.dz, sp.E, e.region, e.primary, clo, chi, e.job, 0u, 0u,
-                 lanemask_lt, nover);
+    const float cpx = e.px, cpy = e.py, cpz = e.pz, csafe = e.safe;
+    const int creg = e.region;
+    const uint32_t cprim = e.primary, cjob = e.job;
     /* regroup: an electron past its step budget for this launch continues in the next one */
     const bool requeue = stepped && res == STEP_CONTINUE && G.steps_per_launch > 0 &&
                          (int)(e.steps - steps0) >= G.steps_per_launch;
@@ -765,7 +775,8 @@ k_transport(DevScene S, GenDev G) {
         }
       }
     }
-    if (stepped && (res != STEP_CONTINUE || requeue)) {
+    const bool ends = stepped && (res != STEP_CONTINUE || requeue);
+    if (ends) {
       alive = false;
       if (res == STEP_ESCAPED) nesc++;
       if (res == STEP_WATCHDOG) {
@@ -782,6 +793,55 @@ k_transport(DevScene S, GenDev G) {
         }
       }
     }
+    /* new secondaries: the k-th spawning lane hands its secondary to the
+     * k-th lane without an electron; the rest go to the queue for the next
+     * launch.  An electron's stream depends only on its lineage, so which
+     * lane runs it changes nothing. */
+    bool queue_spawn = sp.has;
+    if (G.handoff) {
+      const unsigned long long mg = wave_ballot(sp.has);
+      const unsigned long long mt = wave_ballot(!alive);
+      if (mg && mt) {
+        const int n = min(__popcll(mg), __popcll(mt));
+        const int grank = __popcll(mg & lanemask_lt);
+        const int trank = __popcll(mt & lanemask_lt);
+        int* giver = lane_map();
+        if (sp.has && grank < n) giver[grank] = lane_id();
+        __builtin_amdgcn_wave_barrier();
+        const bool take = !alive && trank < n;
+        const int src = take ? giver[trank] : lane_id();
+        const float tpx = __shfl(cpx, src), tpy = __shfl(cpy, src), tpz = __shfl(cpz, src);
+        const float tdx = __shfl(sp.dx, src), tdy = __shfl(sp.dy, src), tdz = __shfl(sp.dz, src);
+        const float tE = __shfl(sp.E, src), tsafe = __shfl(csafe, src);
+        const int treg = __shfl(creg, src), tmax = __shfl(maxSteps, src);
+        const uint32_t tprim = __shfl(cprim, src), tlo = __shfl(clo, src), thi = __shfl(chi, src);
+        const uint32_t tjob = __shfl(cjob, src);
+        if (take) {
+          e.px = tpx; e.py = tpy; e.pz = tpz;
+          e.dx = tdx; e.dy = tdy; e.dz = tdz;
+          e.E = tE;
+          e.region = treg;
+          e.primary = tprim;
+          e.lin_lo = tlo;
+          e.lin_hi = thi;
+          e.job = tjob;
+          e.steps = 0;
+          e.nchild = 0;
+          e.safe = tsafe;
+          if (e.job != cur_job) {
+            if (cur_job != 0xFFFFFFFFu) flush_lane_stats(G, cur_job, nsteps, nelec, nesc, nwd, nover);
+            cur_job = e.job;
+          }
+          maxSteps = tmax;
+          steps0 = 0;
+          nelec++;
+          alive = true;
+        }
+        if (grank < n) queue_spawn = false;
+      }
+    }
+    queue_append(G, queue_spawn, cpx, cpy, cpz, sp.dx, sp.dy, sp.dz, sp.E, creg, cprim, clo, chi, cjob, 0u, 0u,
+                 lanemask_lt, nover);
     PT_END(PH_TAIL, t_ta);
   }
   PT_FLUSH();
@@ -2064,6 +2124,7 @@ void run_generation(monsel_ctx_s* C) {
   G.esc_count = counts + 2;
   G.steps_per_launch = C->tuning.steps_per_launch;
   G.geo_batch = C->tuning.geometry_batch > 0 ? C->tuning.geometry_batch : kDefaultGeometryBatch;
+  G.handoff = C->tuning.spawn_handoff >= 0 ? 1 : 0;
   const int block = C->tuning.block_size > 0 ? C->tuning.block_size : 256;
   const int grid = C->n_cu * (C->blocks_per_cu > 0 ? C->blocks_per_cu : 4);
   /* Event markers time the kernels; MONSEL_NO_EVENT_TIMING=1 leaves them out
@@ -2281,7 +2342,7 @@ int monsel_create(int device_ordinal, const monsel_tuning* tuning, monsel_ctx* o
     if (C->tuning.max_jobs_in_flight < 0 || C->tuning.primaries_per_launch < 0 || C->tuning.steps_per_launch < 0 ||
         C->tuning.queue_capacity < 0 || C->tuning.admission_order < 0 || C->tuning.admission_order > 1 ||
         C->tuning.sort_queue < -1 || C->tuning.sort_queue > 0 || C->tuning.geometry_batch < 0 ||
-        C->tuning.geometry_batch > 64)
+        C->tuning.geometry_batch > 64 || C->tuning.spawn_handoff < -1 || C->tuning.spawn_handoff > 0)
       throw MonselError(MONSEL_E_INVALID, "tuning value out of range");
     C->jobtab.ensure(kMaxJobs * sizeof(JobRec));
     C->counts.ensure((4 + kMaxJobs + 2 * kSortKeys) * sizeof(unsigned int));

@@ -53,9 +53,12 @@ def test_mixed_batch_matches_jobs_alone(M):
 @pytest.mark.parametrize("tuning", [dict(steps_per_launch=16), dict(primaries_per_launch=97),
                                     dict(max_jobs_in_flight=2, steps_per_launch=5), dict(block_size=128),
                                     dict(admission_order=1, primaries_per_launch=300), dict(sort_queue=-1),
-                                    dict(geometry_batch=1), dict(geometry_batch=64, steps_per_launch=7)],
+                                    dict(geometry_batch=1), dict(geometry_batch=64, steps_per_launch=7),
+                                    dict(spawn_handoff=-1),
+                                    dict(spawn_handoff=-1, primaries_per_launch=500, steps_per_launch=9)],
                          ids=["regroup16", "admit97", "two-slots-regroup5", "block128", "fewest-left-first",
-                              "unsorted-queue", "geometry-every-step", "geometry-full-wave-regroup7"])
+                              "unsorted-queue", "geometry-every-step", "geometry-full-wave-regroup7",
+                              "no-handoff", "no-handoff-admit500-regroup9"])
 def test_results_independent_of_tuning(M, tuning):
     from cudamonsel_amd.abi import Engine
     jobs = _jobs(M)[:3]
