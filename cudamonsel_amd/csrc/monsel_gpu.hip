@@ -777,6 +777,7 @@ k_transport(DevScene S, GenDev G) {
     }
     const bool ends = stepped && (res != STEP_CONTINUE || requeue);
     if (ends) {
+      PT_END_ELECTRON(e.steps - steps0);
       alive = false;
       if (res == STEP_ESCAPED) nesc++;
       if (res == STEP_WATCHDOG) {
@@ -2817,6 +2818,29 @@ int monsel_phase_timing_read(int device, uint64_t* out, int n) {
     HIP_TRY(hipMemcpyFromSymbol(out, HIP_SYMBOL(g_phase), sizeof(unsigned long long) * 3 * kPhases));
     std::vector<unsigned long long> zero(3 * kPhases, 0ull);
     HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(g_phase), zero.data(), sizeof(unsigned long long) * 3 * kPhases));
+    return MONSEL_OK;
+  });
+}
+
+/* Phase-timing builds only: per-launch step-count histogram of ending
+ * electrons (hist[kStepBuckets]) and the wave spans recorded since the last
+ * call (spans[2 * max_spans], start/end in 100 MHz ticks); returns the
+ * number of spans (also clears both). */
+int monsel_phase_timing_spans(int device, uint64_t* hist, uint64_t* spans, int max_spans, int* n_spans) {
+  if (!hist || !spans || !n_spans || max_spans < 0) return set_err(MONSEL_E_INVALID, "phase_timing_spans arguments");
+  return guarded([&] {
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipDeviceSynchronize());
+    unsigned int n = 0;
+    HIP_TRY(hipMemcpyFromSymbol(&n, HIP_SYMBOL(g_span_count), sizeof(n)));
+    n = std::min<unsigned int>(n, (unsigned int)std::min(max_spans, kMaxSpans));
+    HIP_TRY(hipMemcpyFromSymbol(hist, HIP_SYMBOL(g_step_hist), sizeof(unsigned long long) * kStepBuckets));
+    if (n) HIP_TRY(hipMemcpyFromSymbol(spans, HIP_SYMBOL(g_spans), sizeof(unsigned long long) * 2 * n));
+    *n_spans = (int)n;
+    std::vector<unsigned long long> zero(kStepBuckets, 0ull);
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(g_step_hist), zero.data(), sizeof(unsigned long long) * kStepBuckets));
+    const unsigned int z = 0;
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(g_span_count), &z, sizeof(z)));
     return MONSEL_OK;
   });
 }

@@ -46,6 +46,14 @@ namespace monsel_dev {
 
 constexpr int kPhaseWaves = 4;  /* waves per workgroup (blockDim.x <= 256) */
 __device__ unsigned long long g_phase[kPhases * 3];
+/* electrons ending after 2^k .. 2^(k+1)-1 steps in one launch (k < 24) */
+constexpr int kStepBuckets = 24;
+__device__ unsigned long long g_step_hist[kStepBuckets];
+/* (start, end) of every wave of the launches since the last read, in the
+ * constant-rate wall clock (wall_clock64, 100 MHz) */
+constexpr int kMaxSpans = 1 << 16;
+__device__ unsigned int g_span_count;
+__device__ unsigned long long g_spans[2 * kMaxSpans];
 
 __device__ __forceinline__ unsigned long long* pt_slot() {
   __shared__ unsigned long long acc[kPhaseWaves * kPhases * 3];
@@ -77,6 +85,11 @@ __device__ __forceinline__ void pt_uniform(int ph, int key) {
   }
 }
 
+__device__ __forceinline__ void pt_end_electron(unsigned int steps) {
+  const int k = steps ? min(31 - __clz(steps), kStepBuckets - 1) : 0;
+  atomicAdd(&g_step_hist[k], 1ull);
+}
+
 __device__ __forceinline__ void pt_init() {
   if ((threadIdx.x & 63) == 0) {
     unsigned long long* s = pt_slot();
@@ -84,8 +97,13 @@ __device__ __forceinline__ void pt_init() {
   }
 }
 
-__device__ __forceinline__ void pt_flush() {
+__device__ __forceinline__ void pt_flush(unsigned long long start) {
   if ((threadIdx.x & 63) == 0) {
+    const unsigned int k = atomicAdd(&g_span_count, 1u);
+    if (k < (unsigned int)kMaxSpans) {
+      g_spans[2 * k] = start;
+      g_spans[2 * k + 1] = wall_clock64();
+    }
     unsigned long long* s = pt_slot();
     for (int k = 0; k < kPhases * 3; ++k)
       if (s[k]) atomicAdd(&g_phase[k], s[k]);
@@ -96,8 +114,9 @@ __device__ __forceinline__ void pt_flush() {
 
 #define PT_BEGIN(v) const unsigned long long v = ::monsel_dev::pt_now()
 #define PT_END(ph, v) ::monsel_dev::pt_add((ph), (v))
-#define PT_INIT() ::monsel_dev::pt_init()
-#define PT_FLUSH() ::monsel_dev::pt_flush()
+#define PT_INIT() ::monsel_dev::pt_init(); const unsigned long long pt_start = wall_clock64()
+#define PT_FLUSH() ::monsel_dev::pt_flush(pt_start)
+#define PT_END_ELECTRON(steps) ::monsel_dev::pt_end_electron(steps)
 #define PT_UNIFORM(ph, key) ::monsel_dev::pt_uniform((ph), (key))
 
 #else
@@ -106,6 +125,7 @@ __device__ __forceinline__ void pt_flush() {
 #define PT_END(ph, v)
 #define PT_INIT()
 #define PT_FLUSH()
+#define PT_END_ELECTRON(steps)
 #define PT_UNIFORM(ph, key)
 
 #endif

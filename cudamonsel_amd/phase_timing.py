@@ -28,6 +28,42 @@ def read_counters(lib, device=0):
     return np.array(out[:], dtype=np.float64).reshape(len(PHASES), 3)
 
 
+def read_spans(lib, device=0, max_spans=1 << 16):
+    """(step histogram [24], wave spans [n, 2] in 100 MHz ticks) recorded
+    since the last read."""
+    hist = (ctypes.c_uint64 * 24)()
+    spans = (ctypes.c_uint64 * (2 * max_spans))()
+    n = ctypes.c_int(0)
+    if lib.monsel_phase_timing_spans(ctypes.c_int(device), hist, spans, ctypes.c_int(max_spans), ctypes.byref(n)) != 0:
+        raise RuntimeError("monsel_phase_timing_spans failed")
+    sp = np.array(spans[:2 * n.value], dtype=np.float64).reshape(-1, 2)
+    return np.array(hist[:], dtype=np.float64), sp
+
+
+def launches(spans):
+    """Splits wave spans into launches (a launch's waves start together) and
+    returns per launch (waves, duration ms, wave-slot occupancy, end times of
+    the waves at 50/90/99/100 % as fractions of the duration)."""
+    order = np.argsort(spans[:, 0])
+    sp = spans[order]
+    out = []
+    i = 0
+    while i < len(sp):
+        j = i + 1
+        end = sp[i, 1]
+        while j < len(sp) and sp[j, 0] <= end:
+            end = max(end, sp[j, 1])
+            j += 1
+        w = sp[i:j]
+        t0, t1 = w[:, 0].min(), w[:, 1].max()
+        dur = t1 - t0
+        occ = float(np.sum(w[:, 1] - w[:, 0]) / (len(w) * dur)) if dur > 0 else 1.0
+        q = np.quantile((w[:, 1] - t0) / dur, [0.5, 0.9, 0.99, 1.0]) if dur > 0 else np.ones(4)
+        out.append((len(w), dur / 1e5, occ, q))
+        i = j
+    return out
+
+
 def table(c):
     """Rows (phase, share of loop cycles, cycles per entry, entries, lane utilisation)."""
     loop = c[PHASES.index("claim"), 0] + c[PHASES.index("step"), 0] + c[PHASES.index("tail"), 0]
@@ -50,6 +86,7 @@ def main():
     lib = M.gpu_lib()
     sc.monte.runPixels(centers, per_pixel, 50, 50.0)   # warm-up
     read_counters(lib)
+    read_spans(lib)
     steps = 0
     for _ in range(repeats):
         r = sc.monte.runPixels(centers, per_pixel, 50, 50.0)
@@ -59,6 +96,12 @@ def main():
     print(f"{'phase':<10} {'share':>7} {'cyc/entry':>10} {'entries':>12} {'lanes':>6}")
     for name, share, cpe, ent, util in table(c):
         print(f"{name:<10} {share:7.3f} {cpe:10.0f} {ent:12d} {util:6.3f}")
+    hist, spans = read_spans(lib)
+    print("steps per electron and launch (log2 buckets):",
+          " ".join(f"{k}:{int(v)}" for k, v in enumerate(hist) if v))
+    print(f"{'launch':>6} {'waves':>6} {'ms':>8} {'occupancy':>9}  wave ends at (50/90/99/100 %)")
+    for k, (nw, ms, occ, q) in enumerate(launches(spans)):
+        print(f"{k:6d} {nw:6d} {ms:8.3f} {occ:9.3f}  " + " ".join(f"{x:.2f}" for x in q))
 
 
 if __name__ == "__main__":
