@@ -47,6 +47,7 @@ class Tuning(ctypes.Structure):
         ("primaries_per_launch", ctypes.c_int32), ("steps_per_launch", ctypes.c_int32),
         ("admission_order", ctypes.c_int32), ("sort_queue", ctypes.c_int32),
         ("geometry_batch", ctypes.c_int32), ("spawn_handoff", ctypes.c_int32),
+        ("tail_requeue", ctypes.c_int32),
     ]
 
 

@@ -138,7 +138,13 @@ struct GenDev {
   int steps_per_launch;      /* regroup interval: steps before an electron returns to the queue (0 = none) */
   int geo_batch;             /* lanes of a wave that wait for findEndOfStep before the wave runs it */
   int handoff;               /* new secondaries go to idle lanes of their wave first */
+  int tail_lanes;            /* tail compaction threshold (0 = off) */
 };
+
+/* launches with fewer items than this run their tail out (compaction would
+ * only pass a few electrons from launch to launch) */
+constexpr uint32_t kTailMinItems = 65536;
+constexpr int kDefaultTailLanes = 32;
 
 /* per-wave LDS table of the handoff: lane of the k-th spawning lane */
 __device__ __forceinline__ int* lane_map() {
@@ -699,7 +705,6 @@ k_transport(DevScene S, GenDev G) {
           } else {
             maxSteps = J.max_steps;
             steps0 = e.steps;
-            if (e.steps == 0) nelec++;
             /* a queued electron is located again (the check opening its next step) */
             alive = idx < G.n_prim ? e.region >= 0 : locate(S, G.jobs, e, e.region);
           }
@@ -729,6 +734,9 @@ k_transport(DevScene S, GenDev G) {
       res = finish_step<TAB>(S, G.jobs, e, sp, maxSteps, st);
       pending = false;
       nsteps++;
+      /* an electron is counted once, at its first step (it may be queued
+       * again before taking one) */
+      if (e.steps == 1u) nelec++;
       PT_END(PH_STEP, t_st);
     }
     PT_BEGIN(t_ta);
@@ -835,7 +843,6 @@ k_transport(DevScene S, GenDev G) {
           }
           maxSteps = tmax;
           steps0 = 0;
-          nelec++;
           alive = true;
         }
         if (grank < n) queue_spawn = false;
@@ -843,6 +850,19 @@ k_transport(DevScene S, GenDev G) {
     }
     queue_append(G, queue_spawn, cpx, cpy, cpz, sp.dx, sp.dy, sp.dz, sp.E, creg, cprim, clo, chi, cjob, 0u, 0u,
                  lanemask_lt, nover);
+    /* tail compaction: once the launch's own items are all claimed, a wave
+     * down to tail_lanes live lanes queues them for the next launch (which
+     * packs them into full waves) and leaves.  A step begun but not finished
+     * is redone there from the same state and draws. */
+    if (G.tail_lanes > 0) {
+      const unsigned long long ma = wave_ballot(alive);
+      if (ma && __popcll(ma) <= G.tail_lanes && wave_ballot(done) != 0ull) {
+        queue_append(G, alive, e.px, e.py, e.pz, e.dx, e.dy, e.dz, e.E, e.region, e.primary, e.lin_lo, e.lin_hi,
+                     e.job, e.steps, e.nchild, lanemask_lt, nover);
+        alive = false;
+        pending = false;
+      }
+    }
     PT_END(PH_TAIL, t_ta);
   }
   PT_FLUSH();
@@ -2126,6 +2146,9 @@ void run_generation(monsel_ctx_s* C) {
   G.steps_per_launch = C->tuning.steps_per_launch;
   G.geo_batch = C->tuning.geometry_batch > 0 ? C->tuning.geometry_batch : kDefaultGeometryBatch;
   G.handoff = C->tuning.spawn_handoff >= 0 ? 1 : 0;
+  G.tail_lanes = C->tuning.tail_requeue < 0 || nprim + C->in_count < kTailMinItems
+                     ? 0
+                     : (C->tuning.tail_requeue > 0 ? C->tuning.tail_requeue : kDefaultTailLanes);
   const int block = C->tuning.block_size > 0 ? C->tuning.block_size : 256;
   const int grid = C->n_cu * (C->blocks_per_cu > 0 ? C->blocks_per_cu : 4);
   /* Event markers time the kernels; MONSEL_NO_EVENT_TIMING=1 leaves them out
@@ -2343,7 +2366,8 @@ int monsel_create(int device_ordinal, const monsel_tuning* tuning, monsel_ctx* o
     if (C->tuning.max_jobs_in_flight < 0 || C->tuning.primaries_per_launch < 0 || C->tuning.steps_per_launch < 0 ||
         C->tuning.queue_capacity < 0 || C->tuning.admission_order < 0 || C->tuning.admission_order > 1 ||
         C->tuning.sort_queue < -1 || C->tuning.sort_queue > 0 || C->tuning.geometry_batch < 0 ||
-        C->tuning.geometry_batch > 64 || C->tuning.spawn_handoff < -1 || C->tuning.spawn_handoff > 0)
+        C->tuning.geometry_batch > 64 || C->tuning.spawn_handoff < -1 || C->tuning.spawn_handoff > 0 ||
+        C->tuning.tail_requeue < -1 || C->tuning.tail_requeue > 64)
       throw MonselError(MONSEL_E_INVALID, "tuning value out of range");
     C->jobtab.ensure(kMaxJobs * sizeof(JobRec));
     C->counts.ensure((4 + kMaxJobs + 2 * kSortKeys) * sizeof(unsigned int));

@@ -253,7 +253,7 @@ typedef struct monsel_tallies {
 
 typedef struct monsel_job_stats {
   uint64_t primaries;
-  uint64_t electrons;     /* primaries + tracked secondaries */
+  uint64_t electrons;     /* primaries + tracked secondaries that took a step */
   uint64_t steps;         /* takeStep calls */
   uint64_t escapes;
   uint64_t watchdog_drops;
@@ -281,6 +281,9 @@ typedef struct monsel_tuning {
   int32_t spawn_handoff;        /* 0: a new secondary goes straight to an idle lane of the wave that
                                  * spawned it when there is one; -1: every one waits in the queue
                                  * for the next launch */
+  int32_t tail_requeue;         /* once a launch's own items are all claimed, a wave down to this many
+                                 * live lanes queues them for the next launch and leaves
+                                 * (0 = default 32, -1 = never; launches of < 65536 items never do) */
 } monsel_tuning;
 
 typedef struct monsel_ctx_s* monsel_ctx;

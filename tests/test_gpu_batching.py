@@ -69,6 +69,26 @@ def test_results_independent_of_tuning(M, tuning):
             _same(a, b)
 
 
+def test_tail_compaction_keeps_results(M):
+    """A launch large enough for tail compaction (>= 65536 items): waves down
+    to a few live lanes queue them for the next launch; the results do not
+    depend on the threshold."""
+    from cudamonsel_amd import scenes
+    from cudamonsel_amd.abi import Engine
+    sc = scenes.line_on_substrate()
+    cs = sc.monte.compileScene()
+    cen = scenes.scan_centers(np.linspace(-20, 20, 64), [0.0], sc.beam_z)
+    res = []
+    for tuning in (dict(tail_requeue=-1), dict(), dict(tail_requeue=48), dict(tail_requeue=48, geometry_batch=8)):
+        with Engine(**tuning) as eng:
+            r = eng.result(eng.submit(eng.upload(cs), cen, 1100, 500.0, 0.5e-9, 21))
+            assert r["stats"]["queue_overflow"] == 0 and r["stats"]["watchdog_drops"] == 0
+            res.append(r)
+    assert res[2]["stats"]["transport_launches"] > res[0]["stats"]["transport_launches"]
+    for r in res[1:]:
+        _same(res[0], r)
+
+
 def test_many_small_jobs_of_uneven_size(M):
     """64 single-pixel jobs, 1e2..1e4 primaries, 0.2..5 keV, three materials,
     flat or step edge, all in flight together; each matches its run alone."""
