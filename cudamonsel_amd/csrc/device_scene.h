@@ -47,21 +47,18 @@ struct DPrim {
   float len2;    /* |axv|^2 */
 };
 
-/* Shapes of at most this many primitives carry the reachability table below. */
+/* Shapes of at most this many primitives (whose truth table is one 32-bit
+ * word) get the sharper safety bound of shape_safety. */
 constexpr int kMaxCstPrims = 5;
 
 /* A region shape: a boolean function of up to kMaxPrims primitives, stored
- * as a 256-entry truth table indexed by the primitives' inside bits.
- * cst[F] bit B (shapes of <= kMaxCstPrims primitives): the truth value is
- * the same at every inside-bit pattern B ^ S, S a subset of F -- crossing any
- * of the primitives in F from pattern B cannot cross the shape's surface. */
+ * as a 256-entry truth table indexed by the primitives' inside bits. */
 struct DShape {
   int first_prim;
   int n_prims;
   int is_sphere; /* the chamber sphere: a single primitive */
-  int has_cst;   /* cst[] is filled */
+  int pad;
   uint32_t truth[8];
-  uint32_t cst[1 << kMaxCstPrims];
 };
 
 struct DRegion {

@@ -144,6 +144,8 @@ struct GenDev {
 /* launches with fewer items than this run their tail out (compaction would
  * only pass a few electrons from launch to launch) */
 constexpr uint32_t kTailMinItems = 65536;
+/* items a wave claims at a time (kept in reserve for its idle lanes) */
+constexpr uint32_t kClaimChunk = 64;
 constexpr int kDefaultTailLanes = 32;
 
 /* per-wave LDS table of the handoff: lane of the k-th spawning lane */
@@ -326,6 +328,27 @@ __device__ __forceinline__ float* lane_esig() {
   return l_esig + threadIdx.x;
 }
 
+/* Per-lane anchor of the safety bound: e.safe bounds the distance to the
+ * surfaces from this point, so from the electron's position it is at least
+ * e.safe - |p - anchor| (triangle inequality).  A random walk drifts from
+ * its anchor more slowly than its path grows, so the bound outlasts one
+ * shrunk by every step. */
+__device__ __forceinline__ float* lane_anchor() {
+  __shared__ float l_anchor[3 * kLaneBlock];
+  return l_anchor + threadIdx.x;
+}
+__device__ __forceinline__ void set_anchor(float x, float y, float z) {
+  float* a = lane_anchor();
+  a[0] = x;
+  a[kLaneBlock] = y;
+  a[2 * kLaneBlock] = z;
+}
+__device__ __forceinline__ float from_anchor(float x, float y, float z) {
+  const float* a = lane_anchor();
+  const float dx = x - a[0], dy = y - a[kLaneBlock], dz = z - a[2 * kLaneBlock];
+  return __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
+}
+
 /* What the start of a step leaves for its end. */
 struct StepStart {
   float freePath;
@@ -337,9 +360,10 @@ struct StepStart {
 /* MonteCarloSS::takeStep (MonteCarloSS.cu:107-163), first half: the
  * mechanism rates and the free path (MONSEL_MaterialScatterModel::
  * randomMeanPathLength, :64-91).  The electron is located (its region
- * contains it).  A free path that ends at least a nudge short of e.safe
+ * contains it).  A free path that ends at least a nudge short of the
+ * safety bound (lane_anchor; refreshed at the current point when too small)
  * cannot reach a boundary; every other step needs the geometry. */
-__device__ __forceinline__ void begin_step_in(const DevScene& S, int region, const Elec& e, StepStart& st) {
+__device__ __forceinline__ void begin_step_in(const DevScene& S, int region, Elec& e, StepStart& st) {
   PT_BEGIN(t_ra);
   const MONSEL_CS DMSM& msm = S.msms[S.regions[region].msm];
   float u[4];
@@ -382,14 +406,23 @@ __device__ __forceinline__ void begin_step_in(const DevScene& S, int region, con
   st.u1 = u[1];
   st.u2 = u[2];
   st.u3 = u[3];
-  st.geo = !(freePath + nudge_nm(e.px, e.py, e.pz) <= e.safe);
+  const float margin = nudge_nm(e.px, e.py, e.pz);
+  st.geo = !(e.safe > 0.f && from_anchor(e.px, e.py, e.pz) + freePath + margin <= e.safe);
   PT_END(PH_RATES, t_ra);
+  if (st.geo && !msm.is_null) {
+    /* the bound at the current point often still clears the step */
+    PT_BEGIN(t_sf);
+    e.safe = region_safety(S, region, e.px, e.py, e.pz);
+    set_anchor(e.px, e.py, e.pz);
+    st.geo = !(freePath + margin <= e.safe);
+    PT_END(PH_SAFETY, t_sf);
+  }
 }
 
 /* The lanes of a wave take their turn by region (one pass per region present,
  * normally one): within a pass the region index is wave-uniform, so the
  * scene records it selects are read with scalar loads. */
-__device__ __forceinline__ void begin_step(const DevScene& S, const Elec& e, StepStart& st) {
+__device__ __forceinline__ void begin_step(const DevScene& S, Elec& e, StepStart& st) {
   PT_UNIFORM(PH_RATES_UNI, e.region);
   for (;;) {
     const int r0 = __builtin_amdgcn_readfirstlane(e.region);
@@ -435,13 +468,14 @@ __device__ __forceinline__ int finish_step_in(const DevScene& S, const JobRec* j
     se.next = region;
     se.boundary = false;
     se.winner = -1;
-    se.safe = e.safe - freePath;
+    se.safe = e.safe;   /* the anchored bound stands */
   }
   float len = se.t;
   e.px = px + len * e.dx;
   e.py = py + len * e.dy;
   e.pz = pz + len * e.dz;
   e.safe = se.boundary ? 0.f : se.safe;
+  if (st.geo && !se.boundary) set_anchor(e.px, e.py, e.pz);  /* the bound is the end point's */
   if (se.boundary && se.next == region) {
     /* a grazing crossing: the point a nudge beyond the boundary is back in
      * this region (a corner clipped within the nudge, or FP32 rounding
@@ -492,7 +526,7 @@ __device__ __forceinline__ int finish_step_in(const DevScene& S, const JobRec* j
       }
       if (trapped || e.E < msm.minE) return STEP_DEAD;
     }
-    if (!(e.safe > nudge_nm(e.px, e.py, e.pz)) && !locate(S, jobs, e, region)) return STEP_DEAD;
+    if (st.geo && !(e.safe > nudge_nm(e.px, e.py, e.pz)) && !locate(S, jobs, e, region)) return STEP_DEAD;
     return STEP_CONTINUE;
   }
   if (se.next >= 0) {
@@ -645,6 +679,7 @@ k_transport(DevScene S, GenDev G) {
   Elec e;
   StepStart st;
   bool alive = false, done = false, pending = false;
+  uint32_t r_base = 0, r_left = 0;     /* wave-uniform reserve of claimed item indices */
   int maxSteps = 0;
   uint32_t steps0 = 0;                 /* step count when the lane took the electron */
   uint32_t cur_job = 0xFFFFFFFFu;      /* job the lane's counters belong to */
@@ -654,16 +689,29 @@ k_transport(DevScene S, GenDev G) {
   PT_INIT();
   for (;;) {
     PT_BEGIN(t_cl);
-    /* claim work for idle lanes: one atomic per wave */
+    /* claim work for idle lanes: from the wave's reserve of claimed items,
+     * topped up a chunk at a time by one atomic per wave */
     bool need = !alive && !done;
     unsigned long long m = wave_ballot(need);
     if (m) {
-      int leader = __ffsll((long long)m) - 1;
-      uint32_t base = 0;
-      if (lane_id() == leader) base = atomicAdd(G.claim, (unsigned int)__popcll(m));
-      base = __shfl(base, leader);
+      const uint32_t cnt = (uint32_t)__popcll(m);
+      uint32_t fresh = 0, want = 0;
+      if (cnt > r_left) {
+        want = max(cnt - r_left, (uint32_t)kClaimChunk);
+        const int leader = __ffsll((long long)m) - 1;
+        if (lane_id() == leader) fresh = atomicAdd(G.claim, want);
+        fresh = __shfl(fresh, leader);
+      }
+      const uint32_t rank = (uint32_t)__popcll(m & lanemask_lt);
+      const uint32_t idx = rank < r_left ? r_base + rank : fresh + (rank - r_left);
+      if (cnt > r_left) {
+        r_base = fresh + (cnt - r_left);
+        r_left = want - (cnt - r_left);
+      } else {
+        r_base += cnt;
+        r_left -= cnt;
+      }
       if (need) {
-        const uint32_t idx = base + (uint32_t)__popcll(m & lanemask_lt);
         if (idx < n_items) {
           if (idx < G.n_prim) {
             /* segment holding item idx (binary search over the running ends) */
@@ -749,7 +797,8 @@ k_transport(DevScene S, GenDev G) {
       clo = (uint32_t)lin;
       chi = (uint32_t)(lin >> 32);
     }
-    const float cpx = e.px, cpy = e.py, cpz = e.pz, csafe = e.safe;
+    const float cpx = e.px, cpy = e.py, cpz = e.pz;
+    const float csafe = sp.has && e.safe > 0.f ? fmaxf(0.f, e.safe - from_anchor(cpx, cpy, cpz)) : 0.f;
     const int creg = e.region;
     const uint32_t cprim = e.primary, cjob = e.job;
     /* regroup: an electron past its step budget for this launch continues in the next one */
@@ -837,6 +886,7 @@ k_transport(DevScene S, GenDev G) {
           e.steps = 0;
           e.nchild = 0;
           e.safe = tsafe;
+          set_anchor(tpx, tpy, tpz);
           if (e.job != cur_job) {
             if (cur_job != 0xFFFFFFFFu) flush_lane_stats(G, cur_job, nsteps, nelec, nesc, nwd, nover);
             cur_job = e.job;
@@ -1638,20 +1688,6 @@ HostScene compile_scene(const monsel_scene_desc* d) {
     }
     for (uint32_t bits = 0; bits < (1u << ds.n_prims); ++bits)
       if (eval_tree(d, s, leaves, bits)) ds.truth[bits >> 5] |= 1u << (bits & 31u);
-    if (ds.n_prims <= kMaxCstPrims) {
-      const uint32_t nb = 1u << ds.n_prims;
-      auto tv = [&](uint32_t b) { return (ds.truth[b >> 5] >> (b & 31u)) & 1u; };
-      for (uint32_t F = 0; F < nb; ++F)
-        for (uint32_t B = 0; B < nb; ++B) {
-          bool same = true;
-          for (uint32_t S = F;; S = (S - 1) & F) { /* every subset of F */
-            if (tv(B ^ S) != tv(B)) same = false;
-            if (S == 0) break;
-          }
-          if (same) ds.cst[F] |= 1u << B;
-        }
-      ds.has_cst = 1;
-    }
     shapeIdx[s] = (int)H.shapes.size();
     H.shapes.push_back(ds);
   }

@@ -1126,16 +1126,30 @@ MD bool is_containing_region(const DevScene& S, int r, int target) {
   return false;
 }
 
+/* T'[S] = T[S ^ B] for the 32 entries of a 5-input truth table */
+MD uint32_t xor_permute(uint32_t T, uint32_t B) {
+  if (B & 1u) T = ((T & 0x55555555u) << 1) | ((T >> 1) & 0x55555555u);
+  if (B & 2u) T = ((T & 0x33333333u) << 2) | ((T >> 2) & 0x33333333u);
+  if (B & 4u) T = ((T & 0x0F0F0F0Fu) << 4) | ((T >> 4) & 0x0F0F0F0Fu);
+  if (B & 8u) T = ((T & 0x00FF00FFu) << 8) | ((T >> 8) & 0x00FF00FFu);
+  if (B & 16u) T = (T << 16) | (T >> 16);
+  return T;
+}
+
 /* Lower bound on the distance from p to the surface of shape sh.  Moving
  * less than the distance D_k to primitive k's surface cannot change k's
- * inside bit; taking the primitives in increasing D_k, the bound is the
- * first D_k at which the bits that may have changed can change the shape's
- * truth value (cst table), so surfaces inside the shape, or shared by
- * primitives whose crossing does not matter, do not limit it.  Shapes
- * without the table: the nearest primitive surface. */
+ * inside bit, so within distance D_k only the bits of the primitives with
+ * D_j <= D_k can have flipped; the bound is the least D_k for which some
+ * flip of those bits changes the shape's truth value.  Surfaces inside the
+ * shape, or shared by primitives whose crossing does not matter, therefore
+ * do not limit it.  With at most kMaxCstPrims primitives the test is on
+ * bit masks: M[S] = truth(B ^ S) != truth(B) over the 32 flip sets S, and
+ * the flip sets within a set F of primitives are the AND, over the
+ * primitives b outside F, of the sets with bit b clear.  Larger shapes: the
+ * nearest primitive surface. */
 MD float shape_safety(const DevScene& S, const MONSEL_CS DShape& sh, float px, float py, float pz) {
   const int n = sh.n_prims;
-  if (!sh.has_cst) {
+  if (n > kMaxCstPrims) {
     float d = kInf;
     for (int k = 0; k < n; ++k) d = fminf(d, fabsf(prim_gmax(S, S.prims[sh.first_prim + k], px, py, pz)));
     return d;
@@ -1151,22 +1165,19 @@ MD float shape_safety(const DevScene& S, const MONSEL_CS DShape& sh, float px, f
       if (g <= 0.f) bits |= 1u << k;
     }
   }
-  uint32_t F = 0;
+  const uint32_t Tp = xor_permute(sh.truth[0], bits);
+  const uint32_t M = Tp ^ (0u - (Tp & 1u));
+  constexpr uint32_t kClear[kMaxCstPrims] = {0x55555555u, 0x33333333u, 0x0F0F0F0Fu, 0x00FF00FFu, 0x0000FFFFu};
+  float best = kInf;
 #pragma unroll
-  for (int m = 0; m < kMaxCstPrims; ++m) {
-    float dmin = kInf;
-    int kmin = -1;
+  for (int k = 0; k < kMaxCstPrims; ++k) {
+    uint32_t within = 0xFFFFFFFFu;
 #pragma unroll
-    for (int k = 0; k < kMaxCstPrims; ++k)
-      if (!((F >> k) & 1u) && k < n && d[k] <= dmin) {
-        dmin = d[k];
-        kmin = k;
-      }
-    if (kmin < 0) break;
-    F |= 1u << kmin;
-    if (!((sh.cst[F] >> bits) & 1u)) return dmin;
+    for (int b = 0; b < kMaxCstPrims; ++b)
+      if (!(d[b] <= d[k])) within &= kClear[b];
+    if (k < n && (M & within)) best = fminf(best, d[k]);
   }
-  return kInf;
+  return best;
 }
 
 /* Lower bound on the distance from p to the surfaces findEndOfStep tests

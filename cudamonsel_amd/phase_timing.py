@@ -14,7 +14,7 @@ import sys
 import numpy as np
 
 PHASES = ["claim", "contain", "rates", "geom", "csd", "select", "elastic", "fitted", "polaron", "phonon",
-          "tabulated", "barrier", "step", "tail", "rates_uni", "geom_uni"]
+          "tabulated", "barrier", "step", "tail", "rates_uni", "geom_uni", "safety"]
 
 
 def read_counters(lib, device=0):
