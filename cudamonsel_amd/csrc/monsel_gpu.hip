@@ -144,8 +144,12 @@ struct GenDev {
 /* launches with fewer items than this run their tail out (compaction would
  * only pass a few electrons from launch to launch) */
 constexpr uint32_t kTailMinItems = 65536;
-/* items a wave claims at a time (kept in reserve for its idle lanes) */
-constexpr uint32_t kClaimChunk = 64;
+/* items a wave claims at a time (kept in reserve for its idle lanes);
+ * 4 to 24 measure alike on the line scan, 128 and more starve other waves */
+#ifndef MONSEL_CLAIM_CHUNK
+#define MONSEL_CLAIM_CHUNK 16
+#endif
+constexpr uint32_t kClaimChunk = MONSEL_CLAIM_CHUNK;
 constexpr int kDefaultTailLanes = 32;
 
 /* per-wave LDS table of the handoff: lane of the k-th spawning lane */
