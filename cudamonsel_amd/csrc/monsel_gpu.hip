@@ -75,6 +75,14 @@ constexpr int kQU = 7; /* u32 fields: region, primary, lineage lo/hi, job slot, 
 constexpr int kEF = 7; /* float fields per escape record: tally origin, d, E */
 constexpr int kEU = 2; /* u32 fields: primary, job slot << 1 | through-the-chamber-wall flag */
 constexpr int kStatSteps = 0, kStatElectrons = 1, kStatEscapes = 2, kStatWatchdog = 3, kStatOverflow = 4;
+/* a job's counters are kept in stripes of 8 u64 (waves add to the stripe of
+ * their index), so that waves of many blocks do not contend for one address;
+ * the host sums the stripes */
+constexpr int kStatStripes = 64;
+
+__device__ __forceinline__ unsigned long long* stat_stripe(unsigned long long* st) {
+  return st + 8 * (((blockIdx.x * (blockDim.x >> 6)) + (threadIdx.x >> 6)) % kStatStripes);
+}
 constexpr int kMaxJobs = 1024; /* job-table slots of one context */
 /* Back-escape energies are summed as integers in units of 2^-20 eV, so that
  * the sums are exact and independent of the order of the atomics. */
@@ -664,7 +672,7 @@ __device__ __forceinline__ void queue_append(const GenDev& G, bool has, float px
 /* adds one lane's counters to its job's statistics (per-lane atomics) */
 __device__ __forceinline__ void flush_lane_stats(const GenDev& G, uint32_t job, uint32_t& nsteps, uint32_t& nelec,
                                                  uint32_t& nesc, uint32_t& nwd, uint32_t& nover) {
-  unsigned long long* st = G.jobs[job].stats;
+  unsigned long long* st = stat_stripe(G.jobs[job].stats);
   if (nsteps) atomicAdd(&st[kStatSteps], (unsigned long long)nsteps);
   if (nelec) atomicAdd(&st[kStatElectrons], (unsigned long long)nelec);
   if (nesc) atomicAdd(&st[kStatEscapes], (unsigned long long)nesc);
@@ -935,7 +943,7 @@ k_transport(DevScene S, GenDev G) {
       wover += __shfl_down(wover, off);
     }
     if (lane_id() == 0) {
-      unsigned long long* st = G.jobs[j0].stats;
+      unsigned long long* st = stat_stripe(G.jobs[j0].stats);
       atomicAdd(&st[kStatSteps], wsteps);
       atomicAdd(&st[kStatElectrons], welec);
       if (wesc) atomicAdd(&st[kStatEscapes], wesc);
@@ -1997,14 +2005,14 @@ void setup_job(monsel_ctx_s* C, Job& job) {
   job.prim_energy.ensure(n * 8);
   job.px.ensure((size_t)np * 6 * 8);
   job.hist.ensure((size_t)hist_len(nbins) * 8);
-  job.stats_dev.ensure(8 * 8);
+  job.stats_dev.ensure(8 * 8 * kStatStripes);
   job.centers_dev.ensure((size_t)np * 3 * 8);
   HIP_TRY(hipMemsetAsync(job.prim_counts.p, 0, n * 4, C->stream));
   HIP_TRY(hipMemsetAsync(job.prim_fwd.p, 0, n * 4, C->stream));
   HIP_TRY(hipMemsetAsync(job.prim_energy.p, 0, n * 8, C->stream));
   HIP_TRY(hipMemsetAsync(job.px.p, 0, (size_t)np * 6 * 8, C->stream));
   HIP_TRY(hipMemsetAsync(job.hist.p, 0, (size_t)hist_len(nbins) * 8, C->stream));
-  HIP_TRY(hipMemsetAsync(job.stats_dev.p, 0, 8 * 8, C->stream));
+  HIP_TRY(hipMemsetAsync(job.stats_dev.p, 0, 8 * 8 * kStatStripes, C->stream));
   HIP_TRY(hipMemcpyAsync(job.centers_dev.p, job.centers.data(), (size_t)np * 3 * 8, hipMemcpyHostToDevice, C->stream));
   const HostScene& H = job.sd->host;
   JobRec& R = C->jobrec[job.slot];
@@ -2054,11 +2062,12 @@ void complete_job(monsel_ctx_s* C, Job& job) {
                        job.px.as<unsigned long long>());
     HIP_TRY(hipGetLastError());
   }
-  std::vector<unsigned long long> px((size_t)np * 6), h((size_t)hist_len(nbins)), st(8);
+  std::vector<unsigned long long> px((size_t)np * 6), h((size_t)hist_len(nbins)), sts(8 * kStatStripes), st(8, 0ull);
   HIP_TRY(hipMemcpyAsync(px.data(), job.px.p, px.size() * 8, hipMemcpyDeviceToHost, C->stream));
   HIP_TRY(hipMemcpyAsync(h.data(), job.hist.p, h.size() * 8, hipMemcpyDeviceToHost, C->stream));
-  HIP_TRY(hipMemcpyAsync(st.data(), job.stats_dev.p, 8 * 8, hipMemcpyDeviceToHost, C->stream));
+  HIP_TRY(hipMemcpyAsync(sts.data(), job.stats_dev.p, 8 * 8 * kStatStripes, hipMemcpyDeviceToHost, C->stream));
   HIP_TRY(hipStreamSynchronize(C->stream));
+  for (int k = 0; k < 8 * kStatStripes; ++k) st[k % 8] += sts[k];
   job.back.assign(px.begin(), px.begin() + np);
   job.fwd.assign(px.begin() + np, px.begin() + 2 * np);
   job.se.assign(px.begin() + 2 * np, px.begin() + 3 * np);
