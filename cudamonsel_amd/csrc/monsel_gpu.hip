@@ -955,30 +955,42 @@ k_transport(DevScene S, GenDev G) {
   }
 }
 
+/* Job slots whose histograms a tally launch keeps in LDS, with their word
+ * offsets in the dynamic shared memory (off[k] .. off[k+1]). */
+constexpr int kLdsJobs = 16;
+struct LdsJobs {
+  int n;
+  int slot[kLdsJobs];
+  int off[kLdsJobs + 1];
+};
+
 /* Tallies of the escape records of one transport launch (grid-stride over
- * the record count the launch left in *count).  The histograms of job slot
- * `lds_job` accumulate in a block-private LDS copy (dynamic shared memory of
- * hist_len(nbins) words) that is added to the job's histograms at the end;
- * other jobs' records use global atomics. */
+ * the record count the launch left in *count).  The histograms of the jobs
+ * in `lj` accumulate in block-private LDS copies that are added to the
+ * jobs' histograms at the end; other jobs' records use global atomics. */
 __global__ void __launch_bounds__(256) k_tally_escapes(DevScene S, const JobRec* jobs, const float* ef,
                                                        const uint32_t* eu, uint32_t cap, const unsigned int* count,
-                                                       int lds_job) {
+                                                       LdsJobs lj) {
   extern __shared__ unsigned int lh[];
-  const int L = lds_job >= 0 ? hist_len(jobs[lds_job].nbins) : 0;
+  const int L = lj.off[lj.n];
   for (int i = threadIdx.x; i < L; i += blockDim.x) lh[i] = 0u;
   __syncthreads();
   const uint32_t n = min(*count, cap);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const uint32_t w = eu[cap + i];
     const int job = (int)(w >> 1);
+    unsigned int* own = nullptr;
+    for (int k = 0; k < lj.n; ++k)
+      if (lj.slot[k] == job) own = lh + lj.off[k];
     tally_escape(S, jobs[job], eu[i], ef[6 * cap + i], ef[i], ef[cap + i], ef[2 * cap + i], ef[3 * cap + i],
-                 ef[4 * cap + i], ef[5 * cap + i], (w & 1u) != 0u, job == lds_job ? lh : nullptr);
+                 ef[4 * cap + i], ef[5 * cap + i], (w & 1u) != 0u, own);
   }
   __syncthreads();
-  if (L > 0) {
-    unsigned long long* h = jobs[lds_job].hist;
-    for (int i = threadIdx.x; i < L; i += blockDim.x)
-      if (lh[i]) atomicAdd(&h[i], (unsigned long long)lh[i]);
+  for (int k = 0; k < lj.n; ++k) {
+    unsigned long long* h = jobs[lj.slot[k]].hist;
+    const int base = lj.off[k], len = lj.off[k + 1] - base;
+    for (int i = threadIdx.x; i < len; i += blockDim.x)
+      if (lh[base + i]) atomicAdd(&h[i], (unsigned long long)lh[base + i]);
   }
 }
 
@@ -2213,18 +2225,20 @@ void run_generation(monsel_ctx_s* C) {
   HIP_TRY(hipGetLastError());
   if (timing) HIP_TRY(hipEventRecord(C->evk1, C->stream));
   /* escape tallies: the histograms go through LDS when one job is in flight */
-  int lds_job = -1;
-  size_t lds_bytes = 0;
-  if (inflight.size() == 1) {
-    const int nb = C->jobrec[inflight[0]->slot].nbins;
-    lds_bytes = (size_t)hist_len(nb) * 4;
-    if (lds_bytes <= 64 * 1024) lds_job = inflight[0]->slot;
-    else lds_bytes = 0;
+  LdsJobs lj;
+  memset(&lj, 0, sizeof(lj));
+  for (Job* j : inflight) {
+    const int len = hist_len(C->jobrec[j->slot].nbins);
+    if (lj.n == kLdsJobs || (size_t)(lj.off[lj.n] + len) * 4 > 64 * 1024) break;
+    lj.slot[lj.n] = j->slot;
+    lj.off[lj.n + 1] = lj.off[lj.n] + len;
+    ++lj.n;
   }
+  const size_t lds_bytes = (size_t)lj.off[lj.n] * 4;
   g_hip_call = "launch k_tally_escapes";
   hipLaunchKernelGGL(k_tally_escapes, dim3(C->n_cu * 4), dim3(256), lds_bytes, C->stream, C->batch.dev,
                      (const JobRec*)G.jobs, (const float*)G.esc_f, (const uint32_t*)G.esc_u, cap,
-                     (const unsigned int*)(counts + 2), lds_job);
+                     (const unsigned int*)(counts + 2), lj);
   HIP_TRY(hipGetLastError());
   int n_slots = 0;
   for (Job* j : inflight) n_slots = std::max(n_slots, j->slot + 1);
