@@ -6,12 +6,16 @@ corner radius 5 nm (NormalMultiPlaneShape n NormalCylindricalShape, built by
 NShapes::createLine) -- at 0.5 keV with a 0.5 nm Gaussian beam, 128 pixels at
 1 nm pitch across the line, 2000 primaries per pixel (256,000 primaries),
 FP32 transport, FP64 tallies.  The scene and tables are resident on the
-device before the timed region; each step submits the job through the C-ABI
-and waits for the tallies.
+device before the timed region; each step submits one scan job through the
+C-ABI and collects its tallies, with up to --inflight (default 4) jobs on the
+device at once (they share transport launches, as a scan service would run
+them).  The latency of one job alone is reported beside it
+(config.ms_per_job_alone, the last warm-up job).
 
 Prints one JSON line.  `roofline` refers to the dominant kernel
-(k_transport): algorithmic bytes per electron step x steps / summed kernel
-time from HIP events on the engine's stream.  `cpu_baseline` times the CPU
+(k_transport): algorithmic bytes per electron step x electron steps of the
+timed jobs / transport-kernel time of the timed region, from HIP events on
+the engine's stream (monsel_ctx_counters).  `cpu_baseline` times the CPU
 oracle (the FP64 restatement, one worker per host core up to 16) on a
 bounded sample of the same scan.
 """
@@ -150,6 +154,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--per-pixel", type=int, default=2000)
+    ap.add_argument("--inflight", type=int, default=4, help="line-scan jobs on the device at once")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="run one device copy of this size first (HBM-counter calibration under rocprof)")
@@ -168,26 +173,39 @@ def main():
     if args.calib_copy_mb > 0:
         copy_calibration(args.calib_copy_mb << 20)
 
+    from cudamonsel_amd.abi import Engine
     sc, centers = build_scene()
-    monte = sc.monte
-    log("line scan")
+    log(f"line scan, {args.inflight} job(s) in flight")
     n_prim = centers.shape[0] * args.per_pixel
-    for _ in range(args.warmup):
-        monte.runPixels(centers, args.per_pixel, 50, 50.0)
-    stats = []
+    eng = Engine()
+    sid = eng.upload(sc.monte.compileScene())
+    job = lambda k: eng.submit(sid, centers, args.per_pixel, 500.0, 0.5e-9, 1000 + k)
+    alone_ms = []
+    for k in range(args.warmup):
+        t0 = time.perf_counter()
+        eng.result(job(k))
+        alone_ms.append((time.perf_counter() - t0) * 1e3)
+    la0, st0, ms0 = eng.counters()
+    # the timed steps: each submits one scan job through the C-ABI; up to
+    # `inflight` of them are on the device at once (they share launches)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        r = monte.runPixels(centers, args.per_pixel, 50, 50.0)
-        stats.append(r.stats)
+    ids, res = [], []
+    for k in range(args.steps):
+        ids.append(job(args.warmup + k))
+        if len(ids) >= args.inflight:
+            res.append(eng.result(ids.pop(0)))
+    while ids:
+        res.append(eng.result(ids.pop(0)))
     wall = time.perf_counter() - t0
-    steps_total = sum(s.steps for s in stats)
-    k_ms = sum(s.transport_ms for s in stats)
-    launches = sum(s.transport_launches for s in stats)
+    la1, st1, ms1 = eng.counters()
+    eng.close()
+    steps_total, k_ms, launches = st1 - st0, ms1 - ms0, la1 - la0
     # kernel time is zero when event timing is switched off (counter sessions)
     achieved = BYTES_PER_STEP * steps_total / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
     prof = profile_figures()
-    se = r.se.sum() / n_prim
-    bse = (r.back.sum() - r.se.sum()) / n_prim
+    r = res[-1]
+    se = r["se"].sum() / n_prim
+    bse = (r["back"].sum() - r["se"].sum()) / n_prim
     out = {
         "metric": "primary electrons/s, 0.5 keV PMMA line on Si line scan",
         "value": round(n_prim * args.steps / wall, 1),
@@ -211,6 +229,8 @@ def main():
             "electron_steps_per_s": round(steps_total / wall, 1),
             "images_per_hour": round(3600.0 * args.steps / wall, 1),
             "transport_launches_per_step": launches / args.steps,
+            "jobs_in_flight": args.inflight,
+            "ms_per_job_alone": round(alone_ms[-1], 3) if alone_ms else None,
             "mean_SE_yield": round(float(se), 4),
             "mean_BSE_yield": round(float(bse), 4),
             "ref_compat": "reference",

@@ -53,6 +53,7 @@ class Tuning(ctypes.Structure):
 
 SYMBOLS = [
     "monsel_last_error", "monsel_abi_version", "monsel_create", "monsel_destroy", "monsel_device_info",
+    "monsel_ctx_counters",
     "monsel_scene_footprint", "monsel_scene_upload", "monsel_scene_release", "monsel_job_footprint",
     "monsel_job_submit", "monsel_job_poll", "monsel_job_wait", "monsel_job_read", "monsel_job_stats_get",
     "monsel_job_release", "monsel_run", "monsel_probe_rates", "monsel_probe_elastic_angle", "monsel_probe_csd",
@@ -204,6 +205,14 @@ class Engine:
                "monsel_job_submit")
         self._keep[jid.value] = (c.shape[0], int(nbins))
         return jid.value
+
+    def counters(self):
+        """(transport launches, electron steps of completed jobs, transport
+        kernel ms) so far on this context."""
+        la, st, ms = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_double()
+        _check(self.L.monsel_ctx_counters(self.ctx, ctypes.byref(la), ctypes.byref(st), ctypes.byref(ms)),
+               "monsel_ctx_counters")
+        return la.value, st.value, ms.value
 
     def poll(self, job_id):
         done = ctypes.c_int(0)

@@ -1922,6 +1922,9 @@ struct monsel_ctx_s {
   uint32_t qcap = 0;          /* entries per queue buffer */
   int cur = 0;                /* queue holding the next launch's input */
   uint32_t in_count = 0;      /* electrons in q[cur] */
+  /* cumulative over the context's transport launches (read under mu) */
+  uint64_t n_launches = 0, n_steps = 0;
+  double transport_ms = 0.;
   unsigned int* host_count = nullptr; /* pinned: out, esc, per-job counts */
   uint32_t* host_segs = nullptr;      /* pinned: segment staging */
   std::vector<std::shared_ptr<Job>> active;
@@ -2098,6 +2101,10 @@ void complete_job(monsel_ctx_s* C, Job& job) {
   job.hazim.assign(h.begin() + 2 * nb2 + 182, h.begin() + 2 * nb2 + 182 + 362);
   job.stats.primaries = job.total;
   job.stats.steps = st[kStatSteps];
+  {
+    std::lock_guard<std::mutex> lk(C->mu);
+    C->n_steps += st[kStatSteps];
+  }
   job.stats.electrons = st[kStatElectrons];
   job.stats.escapes = st[kStatEscapes];
   job.stats.watchdog_drops = st[kStatWatchdog];
@@ -2280,6 +2287,11 @@ void run_generation(monsel_ctx_s* C) {
   }
   float kms = 0.f;
   if (timing) HIP_TRY(hipEventElapsedTime(&kms, C->evk0, C->evk1));
+  {
+    std::lock_guard<std::mutex> lk(C->mu);
+    C->n_launches += 1;
+    C->transport_ms += kms;
+  }
   for (Job* j : inflight) {
     j->outstanding = inflight.size() > 1 ? C->host_count[4 + j->slot] : C->in_count;
     j->generations += 1;
@@ -2494,6 +2506,15 @@ int monsel_device_info(monsel_ctx ctx, int* n_cu, size_t* hbm_bytes, char* name,
     strncpy(name, ctx->name.c_str(), name_len - 1);
     name[name_len - 1] = 0;
   }
+  return MONSEL_OK;
+}
+
+int monsel_ctx_counters(monsel_ctx ctx, uint64_t* launches, uint64_t* steps, double* transport_ms) {
+  if (!ctx) return set_err(MONSEL_E_INVALID, "NULL context");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (launches) *launches = ctx->n_launches;
+  if (steps) *steps = ctx->n_steps;
+  if (transport_ms) *transport_ms = ctx->transport_ms;
   return MONSEL_OK;
 }
 

@@ -297,6 +297,11 @@ int monsel_abi_version(void);
 int monsel_create(int device_ordinal, const monsel_tuning* tuning, monsel_ctx* out);
 int monsel_destroy(monsel_ctx ctx);
 int monsel_device_info(monsel_ctx ctx, int* n_cu, size_t* hbm_bytes, char* name, size_t name_len);
+/* Cumulative counters of a context: transport launches, electron steps of
+ * completed jobs and transport-kernel time (HIP events on the context's
+ * stream).  No reference counterpart: for throughput measurement when jobs
+ * share launches (a job's own transport_ms counts every launch it was in). */
+int monsel_ctx_counters(monsel_ctx ctx, uint64_t* launches, uint64_t* steps, double* transport_ms);
 
 /* Scenes: an immutable, device-resident copy of the object graph and tables. */
 int monsel_scene_footprint(const monsel_scene_desc* desc, size_t* hbm_bytes);
