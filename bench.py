@@ -145,6 +145,12 @@ def served_leg(duration_s, rate):
         log(f"{name}: {r['wall_s']:.2f} s, p50 {r['p50_s']:.2f} s, p99 {r['p99_s']:.2f} s")
         out[name] = {k: round(v, 3) if isinstance(v, float) else v for k, v in r.items() if k != "latencies"}
     out["images_per_hour_ratio"] = round(out["served"]["images_per_hour"] / out["plain"]["images_per_hour"], 3)
+    # the reference's call pattern (one job per pixel), timed on a sample of pixels per request
+    with Engine() as eng:
+        r = serving.per_pixel_plain(eng, reqs, sample=16, log=None)
+    log(f"per-pixel plain (estimated): {r['wall_s']:.2f} s, p50 {r['p50_s']:.2f} s, p99 {r['p99_s']:.2f} s")
+    out["plain_per_pixel"] = {k: round(v, 3) if isinstance(v, float) else v for k, v in r.items()}
+    out["images_per_hour_ratio_vs_per_pixel"] = round(out["served"]["images_per_hour"] / r["images_per_hour"], 2)
     return out
 
 

@@ -8,7 +8,9 @@
   requests are admitted while earlier ones are on the GPU as long as the
   device-memory footprint of everything admitted fits a budget; each
   request's latency runs from its arrival to its result.  `serve(...,
-  plain=True)` runs the same stream one request at a time.
+  plain=True)` runs the same stream one request at a time;
+  `per_pixel_plain` estimates it run as the reference does, one job per
+  pixel.
 
 Scenes are built with the reference's classes (`scenes.py`); jobs go through
 the C-ABI (`abi.Engine`).  Results of a job are the same alone and in any mix
@@ -172,6 +174,38 @@ class Admission:
     def release(self, nbytes):
         self.used -= nbytes
         self.inflight -= 1
+
+
+def per_pixel_plain(engine, requests, sample=16, log=None):
+    """The reference's call pattern on the GPU: each request as one job per
+    pixel (GaussianBeam::setCenter + runMultipleTrajectories per pixel,
+    LinesOnLayers.cu:350-380), one job at a time.  Timed on `sample` pixels of
+    every request (spread over the image); a request's service time is the
+    mean pixel time x its pixels, and the stream is replayed on those service
+    times in arrival order (one request at a time) for latencies."""
+    cache = SceneCache(engine)
+    for r in requests:
+        _request_scene(cache, r)
+    service = []
+    for r in sorted(requests, key=lambda q: q.arrival):
+        sc, cs, sid = _request_scene(cache, r)
+        cen = raster_centers(r.side, sc.beam_z)
+        idx = np.linspace(0, cen.shape[0] - 1, min(sample, cen.shape[0])).astype(int)
+        t0 = time.perf_counter()
+        for k in idx:
+            engine.result(engine.submit(sid, cen[k:k + 1], r.per_pixel, r.beam_eV, 0.5e-9, 5000 + r.rid))
+        dt = (time.perf_counter() - t0) / len(idx)
+        service.append((r, dt * cen.shape[0]))
+        if log:
+            log(f"per-pixel plain: request {r.rid} ({r.side}x{r.side}), {dt * 1e3:.3f} ms per pixel")
+    t = 0.0
+    lat = []
+    for r, sv in service:
+        t = max(t, r.arrival) + sv
+        lat.append(t - r.arrival)
+    wall = t - min(r.arrival for r in requests)
+    return dict(requests=len(requests), wall_s=wall, p50_s=percentile(lat, 50), p99_s=percentile(lat, 99),
+                images_per_hour=3600.0 * len(requests) / wall, pixels_sampled_per_request=sample)
 
 
 def serve(engine, requests, plain=False, hbm_budget=64 << 30, time_scale=1.0, poll_s=0.0005, log=None):

@@ -51,3 +51,45 @@ def test_raster_centers_pitch_and_centring():
     assert c.shape == (16, 3)
     assert np.allclose(np.unique(np.round(c[:, 0] * 1e9, 6)), [-1.5, -0.5, 0.5, 1.5])
     assert np.allclose(c[:, 2], -7e-8) and np.allclose(c[:, :2].mean(0), 0.0)
+
+
+def test_per_pixel_plain_replays_service_times(monkeypatch):
+    """The per-pixel estimate times `sample` single-pixel jobs per request,
+    scales to the request's pixels and replays the stream one request at a
+    time: with a fake engine whose jobs take a fixed time, the service times
+    and queueing delays follow in closed form."""
+    import time as _time
+
+    class FakeEngine:
+        def __init__(self):
+            self.jobs = 0
+
+        def upload(self, cs):
+            return 1
+
+        def submit(self, *a, **k):
+            self.jobs += 1
+            return self.jobs
+
+        def result(self, jid):
+            return {}
+
+    clock = [0.0]
+    monkeypatch.setattr(serving.time, "perf_counter", lambda: clock[0])
+    eng = FakeEngine()
+    orig = eng.result
+
+    def timed_result(jid):
+        clock[0] += 0.001   # 1 ms per pixel job
+        return orig(jid)
+    eng.result = timed_result
+    monkeypatch.setattr(serving, "_request_scene", lambda cache, r: (type("S", (), {"beam_z": 0.0})(), None, 1))
+    reqs = [serving.Request(arrival=0.0, rid=0, side=32, per_pixel=50, beam_eV=500.0, geometry="lines", geo_seed=0),
+            serving.Request(arrival=0.5, rid=1, side=64, per_pixel=50, beam_eV=500.0, geometry="lines", geo_seed=0)]
+    r = serving.per_pixel_plain(eng, reqs, sample=8)
+    assert eng.jobs == 16
+    s0, s1 = 1024 * 0.001, 4096 * 0.001          # pixels x 1 ms
+    assert abs(r["wall_s"] - (s0 + s1)) < 1e-9        # the second request waits for the first
+    lat = sorted([s0, s0 + s1 - 0.5])
+    assert abs(r["p50_s"] - lat[0]) < 1e-9 and abs(r["p99_s"] - lat[1]) < 1e-9
+    assert abs(r["images_per_hour"] - 3600.0 * 2 / (s0 + s1)) < 1e-6
