@@ -1,9 +1,9 @@
 """Launch-shape autotuner for the batched form.
 
 Coordinate search over the engine's tuning knobs -- workgroup size, regroup
-interval (steps per launch), queue depth (new primaries admitted per launch)
-and jobs in flight -- timing a caller-supplied workload on a fresh context
-for each candidate.  The winner can be stored as JSON and passed to
+interval (steps per launch), queue depth (new primaries admitted per launch),
+jobs in flight, tail-compaction threshold and secondary handoff -- timing a
+caller-supplied workload on a fresh context for each candidate.  The winner can be stored as JSON and passed to
 `abi.Engine(**tuning)`.
 """
 import json
@@ -17,6 +17,8 @@ SPACE = {
     "steps_per_launch": [0, 512, 128],
     "primaries_per_launch": [0, 1 << 20, 1 << 17],
     "max_jobs_in_flight": [0, 256, 64],
+    "tail_requeue": [0, 16, 48, -1],
+    "spawn_handoff": [0, -1],
 }
 
 
