@@ -1126,13 +1126,14 @@ MD bool is_containing_region(const DevScene& S, int r, int target) {
   return false;
 }
 
-/* T'[S] = T[S ^ B] for the 32 entries of a 5-input truth table */
-MD uint32_t xor_permute(uint32_t T, uint32_t B) {
+/* T'[S] = T[S ^ B] for the entries of an n-input truth table (n <= 5,
+ * wave-uniform; B < 2^n) */
+MD uint32_t xor_permute(uint32_t T, uint32_t B, int n) {
   if (B & 1u) T = ((T & 0x55555555u) << 1) | ((T >> 1) & 0x55555555u);
-  if (B & 2u) T = ((T & 0x33333333u) << 2) | ((T >> 2) & 0x33333333u);
-  if (B & 4u) T = ((T & 0x0F0F0F0Fu) << 4) | ((T >> 4) & 0x0F0F0F0Fu);
-  if (B & 8u) T = ((T & 0x00FF00FFu) << 8) | ((T >> 8) & 0x00FF00FFu);
-  if (B & 16u) T = (T << 16) | (T >> 16);
+  if (n > 1 && (B & 2u)) T = ((T & 0x33333333u) << 2) | ((T >> 2) & 0x33333333u);
+  if (n > 2 && (B & 4u)) T = ((T & 0x0F0F0F0Fu) << 4) | ((T >> 4) & 0x0F0F0F0Fu);
+  if (n > 3 && (B & 8u)) T = ((T & 0x00FF00FFu) << 8) | ((T >> 8) & 0x00FF00FFu);
+  if (n > 4 && (B & 16u)) T = (T << 16) | (T >> 16);
   return T;
 }
 
@@ -1165,17 +1166,25 @@ MD float shape_safety(const DevScene& S, const MONSEL_CS DShape& sh, float px, f
       if (g <= 0.f) bits |= 1u << k;
     }
   }
-  const uint32_t Tp = xor_permute(sh.truth[0], bits);
+  const uint32_t Tp = xor_permute(sh.truth[0], bits, n);
   const uint32_t M = Tp ^ (0u - (Tp & 1u));
   constexpr uint32_t kClear[kMaxCstPrims] = {0x55555555u, 0x33333333u, 0x0F0F0F0Fu, 0x00FF00FFu, 0x0000FFFFu};
+  /* flip sets of primitives beyond n never occur */
+  uint32_t exist = 0xFFFFFFFFu;
+#pragma unroll
+  for (int b = 0; b < kMaxCstPrims; ++b)
+    if (b >= n) exist &= kClear[b];
   float best = kInf;
+  /* the loops run over the n primitives only (n is wave-uniform) */
 #pragma unroll
   for (int k = 0; k < kMaxCstPrims; ++k) {
-    uint32_t within = 0xFFFFFFFFu;
+    if (k < n) {
+      uint32_t within = exist;
 #pragma unroll
-    for (int b = 0; b < kMaxCstPrims; ++b)
-      if (!(d[b] <= d[k])) within &= kClear[b];
-    if (k < n && (M & within)) best = fminf(best, d[k]);
+      for (int b = 0; b < kMaxCstPrims; ++b)
+        if (b < n && b != k && !(d[b] <= d[k])) within &= kClear[b];
+      if (M & within) best = fminf(best, d[k]);
+    }
   }
   return best;
 }
