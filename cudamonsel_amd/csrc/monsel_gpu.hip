@@ -419,14 +419,20 @@ __device__ __forceinline__ void begin_step_in(const DevScene& S, int region, Ele
   st.u2 = u[2];
   st.u3 = u[3];
   const float margin = nudge_nm(e.px, e.py, e.pz);
-  st.geo = !(e.safe > 0.f && from_anchor(e.px, e.py, e.pz) + freePath + margin <= e.safe);
+  const float current = e.safe > 0.f ? e.safe - from_anchor(e.px, e.py, e.pz) : 0.f;  /* bound here */
+  st.geo = !(freePath + margin <= current);
   PT_END(PH_RATES, t_ra);
-  if (st.geo && !msm.is_null) {
-    /* the bound at the current point often still clears the step */
+  if (!msm.is_null && wave_ballot(st.geo) != 0ull) {
+    /* the bound at the current point often still clears the step.  The pass
+     * costs the wave the same whichever of its lanes take part, so every
+     * lane of the pass refreshes its bound (and keeps the better one). */
     PT_BEGIN(t_sf);
-    e.safe = region_safety(S, region, e.px, e.py, e.pz);
-    set_anchor(e.px, e.py, e.pz);
-    st.geo = !(freePath + margin <= e.safe);
+    const float fresh = region_safety(S, region, e.px, e.py, e.pz);
+    if (fresh > current) {
+      e.safe = fresh;
+      set_anchor(e.px, e.py, e.pz);
+    }
+    st.geo = !(freePath + margin <= fmaxf(fresh, current));
     PT_END(PH_SAFETY, t_sf);
   }
 }
