@@ -45,7 +45,10 @@ struct DPrim {
   float r2;      /* radius^2 (nm^2) */
   float axv[3];  /* cylinder axis vector end1-end0 (nm), for point containment */
   float len2;    /* |axv|^2 */
+  float4 pl[4];  /* the first planes of a multi-plane primitive again, inline (one
+                  * fewer dependent load for the common primitives) */
 };
+constexpr int kInlinePlanes = 4;
 
 /* Shapes of at most this many primitives (whose truth table is one 32-bit
  * word) get the sharper safety bound of shape_safety. */

@@ -1679,6 +1679,7 @@ HostScene compile_scene(const monsel_scene_desc* d) {
           double n[3] = {pl.normal[0], pl.normal[1], pl.normal[2]};
           double b = (n[0] * pl.point[0] + n[1] * pl.point[1] + n[2] * pl.point[2]) * 1e9;
           H.planes.push_back(make_float4((float)n[0], (float)n[1], (float)n[2], (float)b));
+          if (k < kInlinePlanes) p.pl[k] = H.planes.back();
           maxn = std::max(maxn, sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]));
         }
         /* min over the planes of 1/|n|, rounded down (distance bounds) */

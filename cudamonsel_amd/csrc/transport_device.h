@@ -833,6 +833,11 @@ MD bool plane_tie_break(float nx, float ny, float nz) {
   return true;
 }
 
+/* plane k of a multi-plane primitive */
+MD float4 prim_plane(const DevScene& S, const MONSEL_CS DPrim& pr, int k) {
+  return k < kInlinePlanes ? pr.pl[k] : S.planes[pr.first_plane + k];
+}
+
 /* Parameter interval of the line p + t d inside a convex primitive. */
 MD Interval prim_interval(const DevScene& S, const MONSEL_CS DPrim& pr, float px, float py, float pz, float dx, float dy,
                           float dz) {
@@ -842,7 +847,7 @@ MD Interval prim_interval(const DevScene& S, const MONSEL_CS DPrim& pr, float px
   iv.fin = iv.fout = -4;
   if (pr.kind == PRIM_PLANES) {
     for (int k = 0; k < pr.n_planes; ++k) {
-      float4 pl = S.planes[pr.first_plane + k];
+      float4 pl = prim_plane(S, pr, k);
       float num = pl.x * px + pl.y * py + pl.z * pz - pl.w;
       float den = pl.x * dx + pl.y * dy + pl.z * dz;
       if (den == 0.f) {
@@ -934,7 +939,7 @@ MD Interval prim_interval(const DevScene& S, const MONSEL_CS DPrim& pr, float px
 MD void prim_normal(const DevScene& S, const MONSEL_CS DPrim& pr, int f, float qx, float qy, float qz, float& nx, float& ny,
                     float& nz) {
   if (pr.kind == PRIM_PLANES) {
-    float4 pl = S.planes[pr.first_plane + (f < 0 ? 0 : f)];
+    float4 pl = prim_plane(S, pr, f < 0 ? 0 : f);
     nx = pl.x;
     ny = pl.y;
     nz = pl.z;
@@ -965,7 +970,7 @@ MD bool prim_contains(const DevScene& S, const MONSEL_CS DPrim& pr, float px, fl
   if (pr.kind == PRIM_PLANES) {
     /* MultiPlaneShape.cu:23-27 for every plane */
     for (int k = 0; k < pr.n_planes; ++k) {
-      float4 pl = S.planes[pr.first_plane + k];
+      float4 pl = prim_plane(S, pr, k);
       if (!(pl.x * px + pl.y * py + pl.z * pz - pl.w <= 0.f)) return false;
     }
     return true;
@@ -989,7 +994,7 @@ MD float prim_gmax(const DevScene& S, const MONSEL_CS DPrim& pr, float px, float
   if (pr.kind == PRIM_PLANES) {
     float g = -kInf;
     for (int k = 0; k < pr.n_planes; ++k) {
-      const float4 pl = S.planes[pr.first_plane + k];
+      const float4 pl = prim_plane(S, pr, k);
       g = fmaxf(g, pl.x * px + pl.y * py + pl.z * pz - pl.w);
     }
     return g * pr.sdscale;
