@@ -72,6 +72,7 @@ struct DRegion {
   int n_children;
   int depth;
   int pad[2];
+  DShape sh;    /* its shape's record again, inline (one fewer dependent load) */
 };
 
 /* JoyLuoNieminenCSD (eV, nm). */

@@ -458,7 +458,7 @@ __device__ __forceinline__ void begin_step(const DevScene& S, Elec& e, StepStart
 __device__ __forceinline__ bool locate(const DevScene& S, const JobRec* jobs, Elec& e, int r) {
   PT_BEGIN(t_co);
   e.safe = 0.f;
-  if (r < 0 || !shape_contains(S, S.regions[r].shape, e.px, e.py, e.pz))
+  if (r < 0 || !shape_contains(S, S.regions[r].sh, e.px, e.py, e.pz))
     e.region = containing_sub_region(S, jobs[e.job].root, e.px, e.py, e.pz);
   PT_END(PH_CONTAIN, t_co);
   return e.region >= 0;
@@ -564,7 +564,7 @@ __device__ __forceinline__ int finish_step_in(const DevScene& S, const JobRec* j
           if (struck == se.winner) {
             nbx = -se.winNx; nby = -se.winNy; nbz = -se.winNz;
           } else {
-            ShapeHit h = shape_first(S, S.regions[struck].shape, px, py, pz, e.dx, e.dy, e.dz, freePath);
+            ShapeHit h = shape_first(S, S.regions[struck].sh, px, py, pz, e.dx, e.dy, e.dz, freePath);
             nbx = -h.nx; nby = -h.ny; nbz = -h.nz;
           }
         } else {
@@ -875,14 +875,17 @@ k_transport(DevScene S, GenDev G) {
      * lane runs it changes nothing. */
     bool queue_spawn = sp.has;
     if (G.handoff) {
-      const unsigned long long mg = wave_ballot(sp.has);
+      /* a secondary whose parent left the chamber (region -1) goes to the
+       * queue, whose containment check at the claim ends it */
+      const bool give = sp.has && creg >= 0;
+      const unsigned long long mg = wave_ballot(give);
       const unsigned long long mt = wave_ballot(!alive);
       if (mg && mt) {
         const int n = min(__popcll(mg), __popcll(mt));
         const int grank = __popcll(mg & lanemask_lt);
         const int trank = __popcll(mt & lanemask_lt);
         int* giver = lane_map();
-        if (sp.has && grank < n) giver[grank] = lane_id();
+        if (give && grank < n) giver[grank] = lane_id();
         __builtin_amdgcn_wave_barrier();
         const bool take = !alive && trank < n;
         const int src = take ? giver[trank] : lane_id();
@@ -913,7 +916,7 @@ k_transport(DevScene S, GenDev G) {
           steps0 = 0;
           alive = true;
         }
-        if (grank < n) queue_spawn = false;
+        if (give && grank < n) queue_spawn = false;
       }
     }
     queue_append(G, queue_spawn, cpx, cpy, cpz, sp.dx, sp.dy, sp.dz, sp.E, creg, cprim, clo, chi, cjob, 0u, 0u,
@@ -1740,6 +1743,7 @@ HostScene compile_scene(const monsel_scene_desc* d) {
     dr.first_child = (int)H.children.size();
     dr.n_children = (int)kids[r].size();
     dr.depth = dr.parent < 0 ? 0 : H.regions[dr.parent].depth + 1;
+    dr.sh = H.shapes[dr.shape];
     for (int c : kids[r]) H.children.push_back(c);
     H.regions.push_back(dr);
   }
@@ -1834,6 +1838,7 @@ HostScene merge_scenes(const std::vector<const HostScene*>& parts, std::vector<i
     for (DRegion r : P->regions) {
       if (r.parent >= 0) r.parent += o_reg;
       r.shape += o_sh;
+      r.sh.first_prim += o_pr;
       r.msm += o_msm;
       r.first_child += o_ch;
       M.regions.push_back(r);

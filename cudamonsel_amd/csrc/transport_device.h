@@ -1011,8 +1011,44 @@ MD float prim_gmax(const DevScene& S, const MONSEL_CS DPrim& pr, float px, float
 
 MD bool truth(const MONSEL_CS DShape& sh, uint32_t bits) { return (sh.truth[bits >> 5] >> (bits & 31u)) & 1u; }
 
-MD bool shape_contains(const DevScene& S, int s, float px, float py, float pz) {
-  const MONSEL_CS DShape& sh = S.shapes[s];
+/* prim_gmax, and in `in` the primitive's inside bit as prim_contains
+ * computes it.  Planes: the same row values.  A sphere or cylinder: the sign
+ * of prim_gmax, except within a band of 1e-5 of the magnitudes involved
+ * around the surface -- far wider than the few ulps by which the two
+ * formulas can round apart -- where prim_contains itself decides. */
+MD float prim_gmax_in(const DevScene& S, const MONSEL_CS DPrim& pr, float px, float py, float pz, bool& in) {
+  if (pr.kind == PRIM_PLANES) {
+    float g = -kInf;
+    bool all = true;
+    for (int k = 0; k < pr.n_planes; ++k) {
+      const float4 pl = prim_plane(S, pr, k);
+      const float v = pl.x * px + pl.y * py + pl.z * pz - pl.w;
+      g = fmaxf(g, v);
+      all = all && v <= 0.f;
+    }
+    in = all;
+    return g * pr.sdscale;
+  }
+  const float wx = px - pr.e0[0], wy = py - pr.e0[1], wz = pz - pr.e0[2];
+  const float r = __builtin_sqrtf(pr.r2);
+  float g, scale;
+  if (pr.kind == PRIM_CYLINDER) {
+    const float apar = wx * pr.ax[0] + wy * pr.ax[1] + wz * pr.ax[2];
+    const float qx = wx - apar * pr.ax[0], qy = wy - apar * pr.ax[1], qz = wz - apar * pr.ax[2];
+    const float q = __builtin_sqrtf(qx * qx + qy * qy + qz * qz);
+    g = fmaxf(q - r, fmaxf(-apar, apar - pr.len));
+    scale = q + fabsf(apar) + pr.len + r;
+  } else {
+    const float w = __builtin_sqrtf(wx * wx + wy * wy + wz * wz);
+    g = w - r;
+    scale = w + r;
+  }
+  in = g <= 0.f;
+  if (fabsf(g) <= 1e-5f * scale) in = prim_contains(S, pr, px, py, pz);
+  return g;
+}
+
+MD bool shape_contains(const DevScene& S, const MONSEL_CS DShape& sh, float px, float py, float pz) {
   uint32_t bits = 0;
   for (int k = 0; k < sh.n_prims; ++k)
     if (prim_contains(S, S.prims[sh.first_prim + k], px, py, pz)) bits |= 1u << k;
@@ -1029,14 +1065,13 @@ struct ShapeHit {
  * then visits the crossings in increasing t, toggling the primitives'
  * inside bits, until the truth table changes value.  Requires
  * blockDim.x <= kLaneBlock. */
-MD ShapeHit shape_first(const DevScene& S, int s, float px, float py, float pz, float dx, float dy, float dz,
-                        float tmax) {
+MD ShapeHit shape_first(const DevScene& S, const MONSEL_CS DShape& sh, float px, float py, float pz, float dx,
+                        float dy, float dz, float tmax) {
   __shared__ float l_tin[kMaxPrims][kLaneBlock];
   __shared__ float l_tout[kMaxPrims][kLaneBlock];
   __shared__ int8_t l_fin[kMaxPrims][kLaneBlock];
   __shared__ int8_t l_fout[kMaxPrims][kLaneBlock];
   const int tid = threadIdx.x;
-  const MONSEL_CS DShape& sh = S.shapes[s];
   ShapeHit h;
   h.t = kInf;
   h.nx = h.ny = h.nz = 0.f;
@@ -1105,13 +1140,13 @@ MD ShapeHit shape_first(const DevScene& S, int s, float px, float py, float pz, 
 
 /* RegionBase::containingSubRegion (RegionBase.cu:50-60) from region r. */
 MD int containing_sub_region(const DevScene& S, int r, float px, float py, float pz) {
-  if (!shape_contains(S, S.regions[r].shape, px, py, pz)) return -1;
+  if (!shape_contains(S, S.regions[r].sh, px, py, pz)) return -1;
   for (int guard = 0; guard < 64; ++guard) {
     const MONSEL_CS DRegion& rg = S.regions[r];
     int next = -1;
     for (int c = 0; c < rg.n_children; ++c) {
       int ch = S.children[rg.first_child + c];
-      if (shape_contains(S, S.regions[ch].shape, px, py, pz)) {
+      if (shape_contains(S, S.regions[ch].sh, px, py, pz)) {
         next = ch;
         break;
       }
@@ -1153,11 +1188,17 @@ MD uint32_t xor_permute(uint32_t T, uint32_t B, int n) {
  * the flip sets within a set F of primitives are the AND, over the
  * primitives b outside F, of the sets with bit b clear.  Larger shapes: the
  * nearest primitive surface. */
-MD float shape_safety(const DevScene& S, const MONSEL_CS DShape& sh, float px, float py, float pz) {
+MD float shape_safety(const DevScene& S, const MONSEL_CS DShape& sh, float px, float py, float pz, bool& inside) {
   const int n = sh.n_prims;
   if (n > kMaxCstPrims) {
     float d = kInf;
-    for (int k = 0; k < n; ++k) d = fminf(d, fabsf(prim_gmax(S, S.prims[sh.first_prim + k], px, py, pz)));
+    uint32_t bits = 0;
+    for (int k = 0; k < n; ++k) {
+      bool in;
+      d = fminf(d, fabsf(prim_gmax_in(S, S.prims[sh.first_prim + k], px, py, pz, in)));
+      if (in) bits |= 1u << k;
+    }
+    inside = truth(sh, bits);
     return d;
   }
   float d[kMaxCstPrims];
@@ -1166,12 +1207,13 @@ MD float shape_safety(const DevScene& S, const MONSEL_CS DShape& sh, float px, f
   for (int k = 0; k < kMaxCstPrims; ++k) {
     d[k] = kInf;
     if (k < n) {
-      const float g = prim_gmax(S, S.prims[sh.first_prim + k], px, py, pz);
-      d[k] = fabsf(g);
-      if (g <= 0.f) bits |= 1u << k;
+      bool in;
+      d[k] = fabsf(prim_gmax_in(S, S.prims[sh.first_prim + k], px, py, pz, in));
+      if (in) bits |= 1u << k;
     }
   }
   const uint32_t Tp = xor_permute(sh.truth[0], bits, n);
+  inside = (Tp & 1u) != 0u;   /* bit 0 of the permuted table is truth(bits) */
   const uint32_t M = Tp ^ (0u - (Tp & 1u));
   constexpr uint32_t kClear[kMaxCstPrims] = {0x55555555u, 0x33333333u, 0x0F0F0F0Fu, 0x00FF00FFu, 0x0000FFFFu};
   /* flip sets of primitives beyond n never occur */
@@ -1196,12 +1238,20 @@ MD float shape_safety(const DevScene& S, const MONSEL_CS DShape& sh, float px, f
 
 /* Lower bound on the distance from p to the surfaces findEndOfStep tests
  * from region r: r's shape and its children's shapes.  A step from p shorter
- * than this cannot end at a boundary. */
+ * than this cannot end at a boundary.  0 when p is outside r's own shape by
+ * the containment test of the step's start (shape_contains): such an
+ * electron is relocated before its next step, as the reference relocates it
+ * (MonteCarloSS.cu:107-115), so a bound measured from outside the shape
+ * never lets it skip that test. */
 MD float region_safety(const DevScene& S, int r, float px, float py, float pz) {
   const MONSEL_CS DRegion& rg = S.regions[r];
-  float d = kInf;
-  for (int c = -1; c < rg.n_children; ++c)
-    d = fminf(d, shape_safety(S, S.shapes[S.regions[c < 0 ? r : S.children[rg.first_child + c]].shape], px, py, pz));
+  bool inside;
+  float d = shape_safety(S, rg.sh, px, py, pz, inside);
+  if (!inside) return 0.f;
+  for (int c = 0; c < rg.n_children; ++c) {
+    bool in_child;
+    d = fminf(d, shape_safety(S, S.regions[S.children[rg.first_child + c]].sh, px, py, pz, in_child));
+  }
   return d;
 }
 
@@ -1236,7 +1286,7 @@ MD StepEnd find_end_of_step(const DevScene& S, int r, float px, float py, float 
    * the nearest crossing found so far */
   for (int c = -1; c < rg.n_children; ++c) {
     const int reg = c < 0 ? r : S.children[rg.first_child + c];
-    ShapeHit h = shape_first(S, S.regions[reg].shape, px, py, pz, dx, dy, dz, c < 0 ? s : fminf(s, t));
+    ShapeHit h = shape_first(S, S.regions[reg].sh, px, py, pz, dx, dy, dz, c < 0 ? s : fminf(s, t));
     if (c < 0) {
       se.ownNx = se.winNx = h.nx;
       se.ownNy = se.winNy = h.ny;

@@ -100,7 +100,9 @@ def test_safety_bounds_the_next_boundary(oracle, geometry):
     safety bound (monsel_probe_safety).  Pins the bound: no segment from a
     point shorter than its bound reaches a boundary, on the device or in the
     FP64 oracle; in the substrate away from every line the bound is the depth
-    (the one surface there); and the bound is not trivially small."""
+    (the one surface there); outside a region's own shape it is 0 (the
+    containment test opening the next step decides there); and the bound is
+    not trivially small."""
     from cudamonsel_amd import scenes
     from cudamonsel_amd.abi import probe_for
     sc = {"line-compat": lambda: scenes.line_on_substrate(compat=True),
@@ -129,6 +131,9 @@ def test_safety_bounds_the_next_boundary(oracle, geometry):
         z = p0[far, 2]
         expect = np.minimum(z, 60.0 - z) if geometry == "holes" else z
         assert far.sum() > 100 and np.allclose(saf[far], expect, rtol=1e-5, atol=1e-4)
+    sub = reg[np.argmax(p0[:, 2])]
+    vac = ok & (reg == reg[np.argmin(p0[:, 2])]) & (reg != sub) & (p0[:, 2] < -1.0)
+    assert vac.sum() > 100 and np.all(pr.safety(sub, p0[vac]) == 0.0)
     k = 4
     sel = ok & (saf > 2e-3) & (saf < 1e6)
     P = np.repeat(p0[sel], k, 0)
