@@ -373,7 +373,7 @@ struct StepStart {
  * mechanism rates and the free path (MONSEL_MaterialScatterModel::
  * randomMeanPathLength, :64-91).  The electron is located (its region
  * contains it).  A free path that ends at least a nudge short of the
- * safety bound (lane_anchor; refreshed at the current point when too small)
+ * safety bound (anchored at lane_anchor, less the distance moved since)
  * cannot reach a boundary; every other step needs the geometry. */
 __device__ __forceinline__ void begin_step_in(const DevScene& S, int region, Elec& e, StepStart& st) {
   PT_BEGIN(t_ra);
@@ -422,19 +422,6 @@ __device__ __forceinline__ void begin_step_in(const DevScene& S, int region, Ele
   const float current = e.safe > 0.f ? e.safe - from_anchor(e.px, e.py, e.pz) : 0.f;  /* bound here */
   st.geo = !(freePath + margin <= current);
   PT_END(PH_RATES, t_ra);
-  if (!msm.is_null && wave_ballot(st.geo) != 0ull) {
-    /* the bound at the current point often still clears the step.  The pass
-     * costs the wave the same whichever of its lanes take part, so every
-     * lane of the pass refreshes its bound (and keeps the better one). */
-    PT_BEGIN(t_sf);
-    const float fresh = region_safety(S, region, e.px, e.py, e.pz);
-    if (fresh > current) {
-      e.safe = fresh;
-      set_anchor(e.px, e.py, e.pz);
-    }
-    st.geo = !(freePath + margin <= fmaxf(fresh, current));
-    PT_END(PH_SAFETY, t_sf);
-  }
 }
 
 /* The lanes of a wave take their turn by region (one pass per region present,
@@ -486,14 +473,27 @@ __device__ __forceinline__ int finish_step_in(const DevScene& S, const JobRec* j
     se.next = region;
     se.boundary = false;
     se.winner = -1;
-    se.safe = e.safe;   /* the anchored bound stands */
   }
   float len = se.t;
   e.px = px + len * e.dx;
   e.py = py + len * e.dy;
   e.pz = pz + len * e.dz;
-  e.safe = se.boundary ? 0.f : se.safe;
-  if (st.geo && !se.boundary) set_anchor(e.px, e.py, e.pz);  /* the bound is the end point's */
+  /* the safety bound: a geometry step's is its end point's; a step within
+   * the bound keeps the anchored one.  Where findEndOfStep found no boundary
+   * for some lane, the wave computes the end point's bound; the pass costs
+   * the wave the same whichever lanes take part, so every lane of it that
+   * crossed no boundary refreshes its bound (and keeps the better one). */
+  if (st.geo || se.boundary) e.safe = 0.f;
+  if (!msm.is_null && wave_ballot(st.geo && !se.boundary) != 0ull && !se.boundary) {
+    PT_BEGIN(t_sf);
+    const float fresh = region_safety(S, region, e.px, e.py, e.pz);
+    const float kept = e.safe > 0.f ? e.safe - from_anchor(e.px, e.py, e.pz) : 0.f;
+    if (fresh > kept) {
+      e.safe = fresh;
+      set_anchor(e.px, e.py, e.pz);
+    }
+    PT_END(PH_SAFETY, t_sf);
+  }
   if (se.boundary && se.next == region) {
     /* a grazing crossing: the point a nudge beyond the boundary is back in
      * this region (a corner clipped within the nudge, or FP32 rounding

@@ -35,7 +35,7 @@ enum Phase {
   PH_TAIL,         /* secondaries, regroup, escape records after the step */
   PH_RATES_UNI,    /* step starts whose lanes share one region (entries only) */
   PH_GEOM_UNI,     /* findEndOfStep passes whose lanes share one region (entries only) */
-  PH_SAFETY,       /* refresh of the safety bound at a step's start */
+  PH_SAFETY,       /* the safety bound at a step's end point */
   kPhases
 };
 

@@ -1263,8 +1263,6 @@ struct StepEnd {
   float ownNx, ownNy, ownNz;   /* current shape's crossing normal (this step) */
   int winner;                  /* child region whose shape won, -1 = own shape */
   float winNx, winNy, winNz;
-  float safe;   /* no boundary: lower bound on the distance from the end point to any surface
-                 * findEndOfStep tests from this region (region_safety); else 0 */
 };
 
 /* Displacement that carries a point across a boundary at coordinate scale |p|. */
@@ -1305,7 +1303,6 @@ MD StepEnd find_end_of_step(const DevScene& S, int r, float px, float py, float 
   if (t <= s) {
     se.t = t;
     se.boundary = true;
-    se.safe = 0.f;
     if (rg.parent < 0 && se.winner < 0) {
       se.next = -1; /* through the chamber wall */
       return se;
@@ -1325,7 +1322,6 @@ MD StepEnd find_end_of_step(const DevScene& S, int r, float px, float py, float 
   se.t = s;
   se.boundary = false;
   se.next = r;
-  se.safe = region_safety(S, r, px + s * dx, py + s * dy, pz + s * dz);
   return se;
 }
 
