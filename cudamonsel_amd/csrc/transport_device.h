@@ -277,8 +277,25 @@ MD float nieminen_factor(float eps) {
  * `pending` marks a second half at depth d.  Below breakE the Nieminen form
  * kE (1 + 1.5 gamma len kE^1.5)^(-2/3) - kE; above it the Joy-Luo loss,
  * halved while it exceeds 10% of the energy. */
+MD float csd_compute_split(const MONSEL_CS DCSD& c, float len, float kE);
+
+/* The common case in straight-line code: one segment, no halving (the
+ * first pass of csd_compute_split, with the same arithmetic). */
 MD float csd_compute(const MONSEL_CS DCSD& c, float len, float kE) {
   if (!c.active || c.nce == 0) return 0.f;
+  if (kE < c.minE || kE <= 0.f) return 0.f;
+  if (kE <= c.breakE) {
+    const float eps = 1.5f * c.gamma * len * kE * __builtin_sqrtf(kE);
+    return kE * nieminen_factor(eps);
+  }
+  float s = 0.f;
+  for (int i = 0; i < c.nce; ++i) s += c.coef[i] * __logf(c.recipJ[i] * kE + c.beta[i]);
+  const float loss = s * len / kE;
+  if (!(loss > 0.1f * kE)) return -loss;
+  return csd_compute_split(c, len, kE);
+}
+
+MD float csd_compute_split(const MONSEL_CS DCSD& c, float len, float kE) {
   float E = kE, total = 0.f;
   uint32_t pending = 0;
   int d = 0;
@@ -1077,7 +1094,7 @@ MD ShapeHit shape_first(const DevScene& S, const MONSEL_CS DShape& sh, float px,
   h.nx = h.ny = h.nz = 0.f;
   uint32_t bits = 0;
   const int n = sh.n_prims;
-  for (int k = 0; k < n; ++k) {
+  auto one = [&](int k) {
     Interval iv = prim_interval(S, S.prims[sh.first_prim + k], px, py, pz, dx, dy, dz);
     if (!(iv.tin < iv.tout)) {
       iv.tin = -kInf;
@@ -1088,14 +1105,22 @@ MD ShapeHit shape_first(const DevScene& S, const MONSEL_CS DShape& sh, float px,
     l_fin[k][tid] = (int8_t)iv.fin;
     l_fout[k][tid] = (int8_t)iv.fout;
     if (iv.tin <= 0.f && iv.tout > 0.f) bits |= 1u << k;
-  }
+  };
+  /* the first four primitives unrolled, so that their records' loads and
+   * interval arithmetic can overlap */
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (k < n) one(k);
+  for (int k = 4; k < n; ++k) one(k);
   bool state = truth(sh, bits);
   float tcur = 0.f;
   for (int it = 0; it < 2 * n; ++it) {
     /* next crossing after tcur and the primitives crossing there */
     float tnext = kInf;
     uint32_t mask = 0;
-    for (int k = 0; k < n; ++k) {
+#pragma unroll
+    for (int k = 0; k < kMaxPrims; ++k) {
+      if (k >= n) break;
       const float a = l_tin[k][tid], b = l_tout[k][tid];
       const float nk = a > tcur ? a : (b > tcur ? b : kInf);
       if (nk < tnext) {
