@@ -161,6 +161,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--per-pixel", type=int, default=2000)
     ap.add_argument("--inflight", type=int, default=4, help="line-scan jobs on the device at once")
+    ap.add_argument("--scan-tuning", default="",
+                    help="JSON object of engine tuning fields for the line scan "
+                         "(default: cudamonsel_amd/data/tuning_linescan.json)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="run one device copy of this size first (HBM-counter calibration under rocprof)")
@@ -183,7 +186,12 @@ def main():
     sc, centers = build_scene()
     log(f"line scan, {args.inflight} job(s) in flight")
     n_prim = centers.shape[0] * args.per_pixel
-    eng = Engine()
+    if args.scan_tuning:
+        scan_tuning = json.loads(args.scan_tuning)
+    else:
+        tfile = os.path.join(ROOT, "cudamonsel_amd", "data", "tuning_linescan.json")
+        scan_tuning = json.load(open(tfile))["tuning"] if os.path.exists(tfile) else {}
+    eng = Engine(**scan_tuning)
     sid = eng.upload(sc.monte.compileScene())
     job = lambda k: eng.submit(sid, centers, args.per_pixel, 500.0, 0.5e-9, 1000 + k)
     alone_ms = []
@@ -236,6 +244,7 @@ def main():
             "images_per_hour": round(3600.0 * args.steps / wall, 1),
             "transport_launches_per_step": launches / args.steps,
             "jobs_in_flight": args.inflight,
+            "engine_tuning": scan_tuning,
             "ms_per_job_alone": round(alone_ms[-1], 3) if alone_ms else None,
             "mean_SE_yield": round(float(se), 4),
             "mean_BSE_yield": round(float(bse), 4),
