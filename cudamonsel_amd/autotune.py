@@ -2,7 +2,8 @@
 
 Coordinate search over the engine's tuning knobs -- workgroup size, regroup
 interval (steps per launch), queue depth (new primaries admitted per launch),
-jobs in flight, tail-compaction threshold and secondary handoff -- timing a
+jobs in flight, tail-compaction threshold, secondary handoff and the lanes a
+wave gathers for findEndOfStep -- timing a
 caller-supplied workload on a fresh context for each candidate.  The winner can be stored as JSON and passed to
 `abi.Engine(**tuning)`.
 """
@@ -19,6 +20,7 @@ SPACE = {
     "max_jobs_in_flight": [0, 256, 64],
     "tail_requeue": [0, 16, 48, -1],
     "spawn_handoff": [0, -1],
+    "geometry_batch": [0, 4, 8],
 }
 
 
