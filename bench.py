@@ -127,20 +127,22 @@ def batched_leg(n_jobs, tuning):
             "results_identical": bool(same)}
 
 
-def served_leg(duration_s, rate):
+def served_leg(duration_s, rate, tuning):
     """The serving configuration on a shortened stream: Poisson arrivals at
     `rate`/s for duration_s s of SEM-image requests (side 32-256 px, 50-500
     primaries/pixel, 0.3-2 keV, random PMMA line arrays or contact holes on
-    Si), continuous batching against one request at a time."""
+    Si), continuous batching against one request at a time, both on the
+    same engine tuning."""
     from cudamonsel_amd import serving
     from cudamonsel_amd.abi import Engine
     reqs = serving.request_stream(rate_per_s=rate, duration_s=duration_s, seed=5)
     out = {"stream": f"Poisson {rate}/s for {duration_s:g} s, {len(reqs)} requests, "
                      f"{sum(r.primaries for r in reqs)} primaries"}
-    log(f"served leg: {out['stream']}")
+    out["tuning"] = tuning
+    log(f"served leg: {out['stream']}, tuning {tuning}")
     for name, plain in (("served", False), ("plain", True)):
         # served: the request with the fewest primaries left gets new launches first
-        with Engine(admission_order=0 if plain else 1) as eng:
+        with Engine(**dict(tuning, admission_order=0 if plain else 1)) as eng:
             r = serving.serve(eng, reqs, plain=plain, log=log)
         log(f"{name}: {r['wall_s']:.2f} s, p50 {r['p50_s']:.2f} s, p99 {r['p99_s']:.2f} s")
         out[name] = {k: round(v, 3) if isinstance(v, float) else v for k, v in r.items() if k != "latencies"}
@@ -170,6 +172,10 @@ def main():
     ap.add_argument("--batch-jobs", type=int, default=512, help="jobs of the batched leg (0: skip)")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="arrival window of the served leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=2.0)
+    ap.add_argument("--serve-tuning", default='{"geometry_batch": 8}',
+                    help="JSON object of engine tuning fields for the served leg, applied to the served and "
+                         "the one-at-a-time engine alike (measured: +5 %% images/h and a lower p99 than "
+                         "the defaults, p50 0.7 -> 1.0 s)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: no multi-GPU path is built (a scan would shard by pixel "
@@ -271,7 +277,8 @@ def main():
         tuning = _json.load(open(tfile))["tuning"] if os.path.exists(tfile) else {}
         out["batched"] = batched_leg(args.batch_jobs, tuning)
     if args.serve_seconds > 0:
-        out["served"] = served_leg(args.serve_seconds, args.serve_rate)
+        out["served"] = served_leg(args.serve_seconds, args.serve_rate,
+                                   json.loads(args.serve_tuning) if args.serve_tuning else {})
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc, centers)
     print(json.dumps(out), flush=True)
