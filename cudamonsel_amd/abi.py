@@ -58,7 +58,8 @@ SYMBOLS = [
     "monsel_job_submit", "monsel_job_poll", "monsel_job_wait", "monsel_job_read", "monsel_job_stats_get",
     "monsel_job_release", "monsel_run", "monsel_probe_rates", "monsel_probe_elastic_angle", "monsel_probe_csd",
     "monsel_probe_find_end_of_step", "monsel_probe_locate", "monsel_probe_nu_table", "monsel_probe_scatter",
-    "monsel_probe_barrier", "monsel_probe_safety",
+    "monsel_probe_barrier", "monsel_probe_safety", "monsel_job_cancel", "monsel_probe_philox",
+    "monsel_probe_hist_bin", "monsel_probe_primaries",
 ]
 
 
@@ -145,6 +146,17 @@ class Probe:
         return dict(dir=out[:, 0:3], E=out[:, 3], has=out[:, 4] > 0, sE=out[:, 5], sdir=out[:, 6:9],
                     trapped=out[:, 9] > 0)
 
+    def primaries(self, centers_m, per_pixel, beam_eV, sigma_m, seed, n, primary_offset=0):
+        """Start positions (nm, n x 3) and regions of the job's first n
+        primaries, as the transport kernel creates them."""
+        c = np.ascontiguousarray(centers_m, dtype=np.float64).reshape(-1, 3)
+        jd = JobDesc(beam_eV * 1.60217653e-19, sigma_m, c.shape[0], int(per_pixel), c.ctypes.data_as(c_double_p),
+                     int(seed), 50, 0, 50.0, int(primary_offset))
+        out = np.zeros((n, 4), np.float32)
+        _check(self.L.monsel_probe_primaries(self.ctx, self.scene, ctypes.byref(jd), int(n), _fp(out)),
+               "probe_primaries")
+        return out[:, :3], out[:, 3].astype(int)
+
     def barrier(self, region_from, region_to, dirs, normals, E_eV, u):
         """Same layout as the oracle's barrier probe (dict of arrays)."""
         d = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)
@@ -156,6 +168,31 @@ class Probe:
         _check(self.L.monsel_probe_barrier(self.ctx, self.scene, int(region_from), int(region_to), n, _fp(d), _fp(nb),
                                            _fp(e), _fp(uu), _fp(out)), "probe_barrier")
         return dict(dir=out[:, 0:3], E=out[:, 3], region=out[:, 4].astype(int), sign=out[:, 5].astype(int))
+
+
+def philox(ctx_address, ctr, key):
+    """Device Philox4x32-10 words (n x 4, uint32) and their FP32 uniforms."""
+    L = gpu_lib()
+    c = np.ascontiguousarray(ctr, dtype=np.uint32).reshape(-1, 4)
+    k = np.ascontiguousarray(key, dtype=np.uint32).reshape(-1, 2)
+    n = c.shape[0]
+    out = np.zeros((n, 4), np.uint32)
+    u = np.zeros((n, 4), np.float32)
+    p32 = ctypes.POINTER(ctypes.c_uint32)
+    _check(L.monsel_probe_philox(ctypes.c_void_p(ctx_address), n, c.ctypes.data_as(p32), k.ctypes.data_as(p32),
+                                 out.ctypes.data_as(p32), _fp(u)), "probe_philox")
+    return out, u
+
+
+def hist_bin(ctx_address, values, lo, hi, nbins, compat):
+    """Device Histogram::bin of each value (int array)."""
+    L = gpu_lib()
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    out = np.zeros(v.size, np.int32)
+    _check(L.monsel_probe_hist_bin(ctypes.c_void_p(ctx_address), v.size, v.ctypes.data_as(c_double_p),
+                                   ctypes.c_double(lo), ctypes.c_double(hi), int(nbins), int(bool(compat)),
+                                   out.ctypes.data_as(c_int_p)), "probe_hist_bin")
+    return out
 
 
 def probe_for(scene_obj, device=0):
@@ -213,6 +250,18 @@ class Engine:
         _check(self.L.monsel_ctx_counters(self.ctx, ctypes.byref(la), ctypes.byref(st), ctypes.byref(ms)),
                "monsel_ctx_counters")
         return la.value, st.value, ms.value
+
+    def cancel(self, job_id):
+        _check(self.L.monsel_job_cancel(self.ctx, ctypes.c_uint64(job_id)), "monsel_job_cancel")
+
+    def wait_status(self, job_id, timeout_s=None):
+        """monsel_job_wait's status (0 = completed, MONSEL_E_CANCELLED = -8, ...)."""
+        us = -1 if timeout_s is None else int(timeout_s * 1e6)
+        return self.L.monsel_job_wait(self.ctx, ctypes.c_uint64(job_id), ctypes.c_int64(us))
+
+    def release(self, job_id):
+        self._keep.pop(job_id, None)
+        _check(self.L.monsel_job_release(self.ctx, ctypes.c_uint64(job_id)), "monsel_job_release")
 
     def poll(self, job_id):
         done = ctypes.c_int(0)

@@ -791,8 +791,13 @@ k_transport(DevScene S, GenDev G) {
         }
       }
     }
-    if (wave_ballot(alive) == 0ull) break;
+    /* the wave leaves only once every lane has run past the launch's items: a
+     * claim round can hand out nothing but dead electrons (queued ones of a
+     * job the host stopped, or primaries that start outside the chamber),
+     * and the items after them must still be claimed */
+    if (wave_ballot(alive || !done) == 0ull) break;
     PT_END(PH_CLAIM, t_cl);
+    if (wave_ballot(alive) == 0ull) continue;
     /* start a step on every lane without one in progress; a step that may
      * reach a boundary waits for the wave's next findEndOfStep pass, which
      * runs once geo_batch lanes wait or no lane could move without it */
@@ -1295,6 +1300,36 @@ __global__ void k_probe_locate(DevScene S, int n, const float* p, int* out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   out[i] = containing_sub_region(S, 0, p[3 * i], p[3 * i + 1], p[3 * i + 2]);
+}
+
+__global__ void k_probe_philox(int n, const uint32_t* ctr, const uint32_t* key, uint32_t* out, float* out_u) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t o[4];
+  philox4x32_10(ctr[4 * i], ctr[4 * i + 1], ctr[4 * i + 2], ctr[4 * i + 3], key[2 * i], key[2 * i + 1], o[0], o[1],
+                o[2], o[3]);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    out[4 * i + k] = o[k];
+    if (out_u) out_u[4 * i + k] = u01(o[k]);
+  }
+}
+
+__global__ void k_probe_hist_bin(int n, const double* val, double lo, double hi, int nbins, int compat, int* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = hist_bin(val[i], lo, hi, nbins, compat != 0);
+}
+
+__global__ void k_probe_primaries(DevScene S, const JobRec* J, int n, float* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Elec e;
+  make_primary(S, *J, 0u, (uint32_t)i, e);
+  out[4 * i] = e.px;
+  out[4 * i + 1] = e.py;
+  out[4 * i + 2] = e.pz;
+  out[4 * i + 3] = (float)e.region;
 }
 
 __global__ void k_probe_nu(DevScene S, int table, int order, int n, const float* x, float* out) {
@@ -1913,7 +1948,10 @@ struct Job {
   std::string error;
   bool done = false;
   std::chrono::steady_clock::time_point t_submit, t_start;
+  /* set by monsel_job_cancel (any thread), acted on by the worker */
+  std::atomic<bool> cancel{false};
   /* engine state (worker thread only) */
+  bool cancelling = false;    /* stopped: drains its queued electrons, then ends */
   std::shared_ptr<SceneDev> sd;
   int slot = -1;
   uint64_t total = 0, launched = 0;
@@ -1977,8 +2015,13 @@ uint64_t splitmix64_host(uint64_t z) {
   return z ^ (z >> 31);
 }
 
+/* lineage of a job's primaries: the seed enters every electron's stream here */
+uint64_t job_lineage(uint64_t seed) { return splitmix64_host(seed ^ 0x5EED5EED5EED5EEDull); }
+
 constexpr size_t kQueueEntryBytes = (size_t)(kQF + kQU) * 4;
 constexpr size_t kEscEntryBytes = (size_t)(kEF + kEU) * 4;
+/* largest queue capacity whose field offsets (up to 7 x capacity) fit u32 */
+constexpr uint32_t kMaxQueueCapacity = 0xFFFFFFFFu / 8u;
 
 /* Queue entries for the device budget: both generation buffers plus the
  * escape records (tuning.queue_capacity overrides). */
@@ -1986,7 +2029,7 @@ uint32_t queue_capacity(monsel_ctx_s* C) {
   if (C->tuning.queue_capacity > 0) return (uint32_t)C->tuning.queue_capacity;
   size_t budget = std::min<size_t>((size_t)6 << 30, C->hbm / 8);
   uint64_t cap = budget / (2 * kQueueEntryBytes + kEscEntryBytes);
-  return (uint32_t)std::min<uint64_t>(cap, 0x7FFFFFF0ull);
+  return (uint32_t)std::min<uint64_t>(cap, kMaxQueueCapacity);
 }
 
 /* Rebuilds the batch scene from the scenes of the active jobs. Only between
@@ -2056,7 +2099,7 @@ void setup_job(monsel_ctx_s* C, Job& job) {
   const HostScene& H = job.sd->host;
   JobRec& R = C->jobrec[job.slot];
   memset(&R, 0, sizeof(R));
-  R.lin0 = splitmix64_host(jd.seed ^ 0x5EED5EED5EED5EEDull);
+  R.lin0 = job_lineage(jd.seed);
   R.E0 = (float)E0eV;
   R.width = (float)(jd.beam_width * 1e9);
   R.centers = job.centers_dev.as<double>();
@@ -2162,6 +2205,16 @@ void release_job(monsel_ctx_s* C, Job& job) {
   job.centers_dev.reset();
   job.wd.reset();
   job.sd.reset();
+}
+
+/* Marks a job's device record so that the transport kernel drops its
+ * electrons when it claims them (counted as lost in its statistics). */
+void stop_job(monsel_ctx_s* C, Job& job) {
+  if (C->jobrec[job.slot].killed) return;
+  C->jobrec[job.slot].killed = 1;
+  HIP_TRY(hipMemcpyAsync(&C->jobtab.as<JobRec>()[job.slot].killed, &C->jobrec[job.slot].killed, sizeof(int),
+                         hipMemcpyHostToDevice, C->stream));
+  HIP_TRY(hipStreamSynchronize(C->stream));
 }
 
 /* One launch of the transport kernel over new primaries of the active jobs
@@ -2283,6 +2336,12 @@ void run_generation(monsel_ctx_s* C) {
   HIP_TRY(hipMemcpyAsync(C->host_count, counts, (4 + std::max(n_slots, 1)) * sizeof(unsigned int),
                          hipMemcpyDeviceToHost, C->stream));
   HIP_TRY(hipStreamSynchronize(C->stream));
+  /* every item of the launch was claimed (waves claim in chunks, so the
+   * counter ends at or beyond the item count) */
+  if ((uint64_t)C->host_count[0] < nprim + (uint64_t)G.in_n)
+    throw MonselError(MONSEL_E_DEVICE, "transport launch ended with unclaimed items (" +
+                                           std::to_string(C->host_count[0]) + " of " +
+                                           std::to_string(nprim + G.in_n) + ")");
   const uint32_t outc = C->host_count[1];
   C->in_count = outc > cap ? cap : outc;
   C->cur = cur ^ 1;
@@ -2324,13 +2383,8 @@ void run_generation(monsel_ctx_s* C) {
     /* an electron still queued has at most K descendants (its energy is below
      * the beam energy), so the reservation shrinks with the queue */
     j->reserved = std::min<uint64_t>(j->reserved, (uint64_t)j->outstanding * j->K);
-    if (j->generations > maxGen && !C->jobrec[j->slot].killed) {
-      /* generation limit: the job's remaining electrons are dropped and counted */
-      C->jobrec[j->slot].killed = 1;
-      HIP_TRY(hipMemcpyAsync(&C->jobtab.as<JobRec>()[j->slot].killed, &C->jobrec[j->slot].killed, sizeof(int),
-                             hipMemcpyHostToDevice, C->stream));
-      HIP_TRY(hipStreamSynchronize(C->stream));
-    }
+    /* generation limit: the job's remaining electrons are dropped and counted */
+    if (j->generations > maxGen) stop_job(C, *j);
   }
 }
 
@@ -2381,15 +2435,24 @@ void worker_main(monsel_ctx_s* C) {
         }
       }
       if (C->active.empty()) continue;
+      /* cancelled jobs: no further primaries; their queued electrons are
+       * dropped when the next launch claims them */
+      for (auto& j : C->active)
+        if (j->cancel.load() && !j->cancelling) {
+          j->cancelling = true;
+          j->launched = j->total;
+          stop_job(C, *j);
+        }
       run_generation(C);
       /* completed jobs: all primaries launched and none of their electrons queued */
       for (size_t k = 0; k < C->active.size();) {
         auto j = C->active[k];
         if (j->launched >= j->total && j->outstanding == 0) {
-          complete_job(C, *j);
+          if (!j->cancelling) complete_job(C, *j);
           release_job(C, *j);
           C->active.erase(C->active.begin() + k);
-          finish_job(C, j, MONSEL_OK, "");
+          if (j->cancelling) finish_job(C, j, MONSEL_E_CANCELLED, "job cancelled");
+          else finish_job(C, j, MONSEL_OK, "");
         } else {
           ++k;
         }
@@ -2474,6 +2537,10 @@ int monsel_create(int device_ordinal, const monsel_tuning* tuning, monsel_ctx* o
     for (int k = kMaxJobs - 1; k >= 0; --k) C->free_slots.push_back(k);
     C->qcap = queue_capacity(C.get());
     if (C->qcap < 65536) throw MonselError(MONSEL_E_INVALID, "queue_capacity below 65536 entries");
+    /* device code addresses the SoA queues and escape records as
+     * field * capacity + slot in 32-bit arithmetic (7 fields) */
+    if (C->qcap > kMaxQueueCapacity)
+      throw MonselError(MONSEL_E_INVALID, "queue_capacity above " + std::to_string(kMaxQueueCapacity) + " entries");
     C->q[0].ensure((size_t)C->qcap * kQueueEntryBytes);
     C->q[1].ensure((size_t)C->qcap * kQueueEntryBytes);
     C->esc.ensure((size_t)C->qcap * kEscEntryBytes);
@@ -2687,6 +2754,29 @@ int monsel_job_release(monsel_ctx ctx, monsel_job_id job) {
   if (it == ctx->jobs.end()) return set_err(MONSEL_E_NOTFOUND, "unknown job");
   if (!it->second->done) return set_err(MONSEL_E_NOTREADY, "job still running");
   ctx->jobs.erase(it);
+  return MONSEL_OK;
+}
+
+int monsel_job_cancel(monsel_ctx ctx, monsel_job_id job) {
+  if (!ctx) return set_err(MONSEL_E_INVALID, "NULL context");
+  std::shared_ptr<Job> waiting;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->jobs.find(job);
+    if (it == ctx->jobs.end()) return set_err(MONSEL_E_NOTFOUND, "unknown job");
+    auto j = it->second;
+    if (j->done) return MONSEL_OK;
+    auto q = std::find(ctx->queue.begin(), ctx->queue.end(), j);
+    if (q != ctx->queue.end()) {
+      /* not admitted yet: it holds nothing on the device */
+      ctx->queue.erase(q);
+      waiting = j;
+    } else {
+      j->cancel.store(true);
+    }
+  }
+  if (waiting) finish_job(ctx, waiting, MONSEL_E_CANCELLED, "job cancelled");
+  ctx->cv_work.notify_one();
   return MONSEL_OK;
 }
 
@@ -2934,6 +3024,77 @@ int monsel_probe_barrier(monsel_ctx ctx, monsel_scene_id scene, int region_from,
                        dN.d, dE.d, dU.d, dO.d);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpy(out, dO.d, 24 * (size_t)n, hipMemcpyDeviceToHost));
+    return MONSEL_OK;
+  });
+}
+
+int monsel_probe_philox(monsel_ctx ctx, int n, const uint32_t* ctr, const uint32_t* key, uint32_t* out,
+                        float* out_u) {
+  if (!ctx) return set_err(MONSEL_E_INVALID, "NULL context");
+  if (n < 0 || (n > 0 && (!ctr || !key || !out))) return set_err(MONSEL_E_INVALID, "probe_philox arguments");
+  if (n == 0) return MONSEL_OK;
+  return guarded([&] {
+    HIP_TRY(hipSetDevice(ctx->device));
+    Scratch<uint32_t> dC(4 * (size_t)n), dK(2 * (size_t)n), dO(4 * (size_t)n);
+    Scratch<float> dU(out_u ? 4 * (size_t)n : 1);
+    HIP_TRY(hipMemcpy(dC.d, ctr, 16 * (size_t)n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dK.d, key, 8 * (size_t)n, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_probe_philox, dim3(grid_for(n)), dim3(256), 0, 0, n, dC.d, dK.d, dO.d,
+                       out_u ? dU.d : nullptr);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(out, dO.d, 16 * (size_t)n, hipMemcpyDeviceToHost));
+    if (out_u) HIP_TRY(hipMemcpy(out_u, dU.d, 16 * (size_t)n, hipMemcpyDeviceToHost));
+    return MONSEL_OK;
+  });
+}
+
+int monsel_probe_hist_bin(monsel_ctx ctx, int n, const double* val, double min, double max, int nbins, int compat,
+                          int* out) {
+  if (!ctx) return set_err(MONSEL_E_INVALID, "NULL context");
+  if (n < 0 || nbins < 1 || !(max > min) || (n > 0 && (!val || !out)))
+    return set_err(MONSEL_E_INVALID, "probe_hist_bin arguments");
+  if (n == 0) return MONSEL_OK;
+  return guarded([&] {
+    HIP_TRY(hipSetDevice(ctx->device));
+    Scratch<double> dV(n);
+    Scratch<int> dO(n);
+    HIP_TRY(hipMemcpy(dV.d, val, 8 * (size_t)n, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_probe_hist_bin, dim3(grid_for(n)), dim3(256), 0, 0, n, dV.d, min, max, nbins, compat,
+                       dO.d);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(out, dO.d, 4 * (size_t)n, hipMemcpyDeviceToHost));
+    return MONSEL_OK;
+  });
+}
+
+int monsel_probe_primaries(monsel_ctx ctx, monsel_scene_id scene, const monsel_job_desc* job, int n, float* out) {
+  std::shared_ptr<SceneDev> sd;
+  int st = probe_prologue(ctx, scene, sd);
+  if (st) return st;
+  st = validate_job(job);
+  if (st) return st;
+  if (n < 0 || (n > 0 && !out) || (uint64_t)n > (uint64_t)job->n_pixels * (uint64_t)job->primaries_per_pixel)
+    return set_err(MONSEL_E_INVALID, "probe_primaries arguments");
+  if (n == 0) return MONSEL_OK;
+  return guarded([&] {
+    Scratch<double> dCen(3 * (size_t)job->n_pixels);
+    Scratch<JobRec> dJ(1);
+    Scratch<float> dO(4 * (size_t)n);
+    HIP_TRY(hipMemcpy(dCen.d, job->pixel_center, 24 * (size_t)job->n_pixels, hipMemcpyHostToDevice));
+    JobRec R;
+    memset(&R, 0, sizeof(R));
+    R.lin0 = job_lineage(job->seed);
+    R.E0 = (float)(job->beam_energy / kE);
+    R.width = (float)(job->beam_width * 1e9);
+    R.centers = dCen.d;
+    R.n_pixels = job->n_pixels;
+    R.ppp = job->primaries_per_pixel;
+    R.prim_base = (uint32_t)job->primary_offset;
+    R.root = 0;
+    HIP_TRY(hipMemcpy(dJ.d, &R, sizeof(R), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_probe_primaries, dim3(grid_for(n)), dim3(256), 0, 0, sd->dev, (const JobRec*)dJ.d, n, dO.d);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(out, dO.d, 16 * (size_t)n, hipMemcpyDeviceToHost));
     return MONSEL_OK;
   });
 }

@@ -48,7 +48,8 @@ typedef enum monsel_status {
   MONSEL_E_DEVICE = -4,      /* HIP runtime error (no device, launch failure, ...) */
   MONSEL_E_NOTFOUND = -5,    /* unknown scene or job id */
   MONSEL_E_TIMEOUT = -6,     /* monsel_job_wait timed out */
-  MONSEL_E_NOTREADY = -7     /* job still running */
+  MONSEL_E_NOTREADY = -7,    /* job still running */
+  MONSEL_E_CANCELLED = -8    /* job ended by monsel_job_cancel: no results */
 } monsel_status;
 
 /* Reference-compatibility switches ("RefCompat").  Each bit reproduces one
@@ -317,6 +318,16 @@ int monsel_job_wait(monsel_ctx ctx, monsel_job_id job, int64_t timeout_us);
 int monsel_job_read(monsel_ctx ctx, monsel_job_id job, monsel_tallies* out);
 int monsel_job_stats_get(monsel_ctx ctx, monsel_job_id job, monsel_job_stats* out);
 int monsel_job_release(monsel_ctx ctx, monsel_job_id job);
+/* Ends a job early.  No reference counterpart (runMultipleTrajectories,
+ * MonteCarloSS.cu:222-231, always runs to its end); a serving loop uses it to
+ * drop a request that timed out.  A job still waiting for admission ends at once; a running one
+ * launches no further primaries, its queued electrons are dropped at the next
+ * transport launch, and then its slot, device buffers and queue reservation
+ * are freed.  Other jobs sharing those launches are unaffected (their results
+ * do not depend on the mix).  wait/read on a cancelled job return
+ * MONSEL_E_CANCELLED; release it as any other.  Cancelling a job that has
+ * already completed changes nothing (returns MONSEL_OK). */
+int monsel_job_cancel(monsel_ctx ctx, monsel_job_id job);
 
 /* Convenience: submit + wait + read + release. */
 int monsel_run(monsel_ctx ctx, monsel_scene_id scene, const monsel_job_desc* job,
@@ -385,6 +396,27 @@ int monsel_probe_scatter(monsel_ctx ctx, monsel_scene_id scene, int msm, int mec
  * (+1 along the normal, -1 against it). */
 int monsel_probe_barrier(monsel_ctx ctx, monsel_scene_id scene, int region_from, int region_to, int n,
                          const float* dir, const float* normal, const float* energies_ev, const float* u, float* out);
+
+/* The device's Philox4x32-10 (the generator that replaces Math2::random,
+ * gov/nist/microanalysis/Utility/Math2.cu:1328-1336, with per-electron
+ * streams): out[4i..4i+3] = Philox4x32-10(counter ctr[4i..4i+3], key
+ * key[2i..2i+1]); out_u (may be NULL) receives the FP32 uniforms in (0, 1)
+ * the transport kernel makes of those words. */
+int monsel_probe_philox(monsel_ctx ctx, int n, const uint32_t* ctr, const uint32_t* key, uint32_t* out,
+                        float* out_u);
+
+/* Histogram::bin (gov/nist/microanalysis/Utility/Histogram.cu:73-79 with
+ * Amphibian/Algorithm.cu:5-14) as the escape tallies compute it: bin index
+ * of val[i] over nbins bins of [min, max], -1 = underrange, nbins =
+ * overrange; compat != 0 selects the reference's insertion-point result. */
+int monsel_probe_hist_bin(monsel_ctx ctx, int n, const double* val, double min, double max, int nbins, int compat,
+                          int* out);
+
+/* GaussianBeam::createElectron (gov/nist/microanalysis/NISTMonte/GaussianBeam.cu:52-65)
+ * for primaries 0 .. n-1 of `job` as the transport kernel starts them (same
+ * random stream, same pixel order): out[4i..4i+3] = position (nm) and the
+ * region containing it (-1 = outside the chamber). */
+int monsel_probe_primaries(monsel_ctx ctx, monsel_scene_id scene, const monsel_job_desc* job, int n, float* out);
 
 #ifdef __cplusplus
 }

@@ -1434,6 +1434,19 @@ struct JobParams {
   uint64_t seed;
 };
 
+/* GaussianBeam::createElectron (GaussianBeam.cu:52-65): the start point
+ * center + r (cos th, sin th, 0), r = width sqrt(-2 ln U1), th = 2 pi U2,
+ * drawn from the electron's stream. */
+static void gaussian_beam_start(const JobParams& jp, const double* center, Stream& rng, double* ipos) {
+  ipos[0] = center[0];
+  ipos[1] = center[1];
+  ipos[2] = center[2];
+  double r = ::sqrt(-2. * ::log(rng.next())) * jp.width;
+  double th = 2.0 * PI * (rng.next());
+  ipos[0] += r * ::cos(th);
+  ipos[1] += r * ::sin(th);
+}
+
 /* MonteCarloSS.cu:99-231 for one primary; returns (nSE, nBack). */
 static void run_trajectory(Ctx& cx, const JobParams& jp, const double* center, uint32_t primary, PixelTally& pt,
                            JobHist& hist, Counters& cnt) {
@@ -1441,12 +1454,8 @@ static void run_trajectory(Ctx& cx, const JobParams& jp, const double* center, u
   std::vector<Electron> stack;
   Electron cur;
   cur.rng.init(jp.seed, 0, primary);
-  /* GaussianBeam.cu:52-65 */
-  double ipos[3] = {center[0], center[1], center[2]};
-  double r = ::sqrt(-2. * ::log(cur.rng.next())) * jp.width;
-  double th = 2.0 * PI * (cur.rng.next());
-  ipos[0] += r * ::cos(th);
-  ipos[1] += r * ::sin(th);
+  double ipos[3];
+  gaussian_beam_start(jp, center, cur.rng, ipos);
   cur.init(ipos, 0., 0., jp.E0);
   cur.setRegion(cx.geo.containing_sub_region(0, ipos));
   cnt.electrons++;
@@ -1845,6 +1854,45 @@ int oracle_run_job(void* scene, const monsel_job_desc* job, monsel_tallies* out,
 }
 
 /* ---- probes (double precision; energies in eV, lengths in nm) ---------- */
+
+/* Start points (m) and regions of primaries 0..n-1 of a job, as
+ * run_trajectory creates them (GaussianBeam.cu:52-65). */
+int oracle_probe_primaries(void* scene, const monsel_job_desc* job, int n, double* out) {
+  if (!scene || !job || n < 0 || (n > 0 && !out) || job->primaries_per_pixel <= 0) return -1;
+  Ctx cx(*(Scene*)scene);
+  JobParams jp{};
+  jp.E0 = job->beam_energy;
+  jp.width = job->beam_width;
+  jp.seed = job->seed;
+  for (int i = 0; i < n; ++i) {
+    const uint32_t primary = (uint32_t)(job->primary_offset + (uint64_t)i);
+    const int pix = i / job->primaries_per_pixel;
+    if (pix >= job->n_pixels) return -1;
+    Stream rng;
+    rng.init(jp.seed, 0, primary);
+    double p[3];
+    gaussian_beam_start(jp, job->pixel_center + 3 * (size_t)pix, rng, p);
+    out[4 * i] = p[0];
+    out[4 * i + 1] = p[1];
+    out[4 * i + 2] = p[2];
+    out[4 * i + 3] = cx.geo.containing_sub_region(0, p);
+  }
+  return 0;
+}
+
+/* TabulatedInelasticSM.cu:103-151 as the oracle evaluates it (J). */
+double oracle_e0_from_dispersion(double Eq, double deltaE) { return compute_E0_from_dispersion(Eq, deltaE); }
+
+/* ExpQMBarrierSM.cu:21-48: transmission probability of a barrier of width
+ * lambda (m; 0 = the sharp step) for perpendicular energy perpE above a step
+ * deltaU (J), as barrier_core computes it; 0 where the classical threshold
+ * reflects. */
+double oracle_barrier_T(double lambda, double perpE, double deltaU) {
+  if (perpE <= 0. || perpE <= deltaU) return 0.;
+  const double rootPerpE = ::sqrt(perpE), rootDiff = ::sqrt(perpE - deltaU);
+  if (lambda == 0.) return sharp_barrier_T(rootPerpE, rootDiff);
+  return general_barrier_T((PI * lambda * ::sqrt(EMASS / 2.)) / HBAR, rootPerpE, rootDiff);
+}
 
 int oracle_probe_rates(void* scene, int msm, int n, const double* e_ev, double* out) {
   const Scene& sc = *(const Scene*)scene;

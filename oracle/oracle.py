@@ -82,6 +82,10 @@ def lib():
         L.oracle_u_d2.restype = ctypes.c_double
         L.oracle_hist_bin.restype = ctypes.c_int
         L.oracle_hist_bin.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.oracle_e0_from_dispersion.restype = ctypes.c_double
+        L.oracle_e0_from_dispersion.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.oracle_barrier_T.restype = ctypes.c_double
+        L.oracle_barrier_T.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double]
         L.oracle_child_lineage.restype = ctypes.c_uint64
         L.oracle_child_lineage.argtypes = [ctypes.c_uint64, ctypes.c_uint32]
         _lib = L
@@ -132,6 +136,17 @@ class OracleScene:
         out["stats"] = {k: getattr(st, k) for k, _ in JobStats._fields_}
         out["perPixel"] = per_pixel
         return out
+
+    def primaries(self, centers, per_pixel, beam_width_m, seed, n, primary_offset=0):
+        """Start points (m, n x 3) and regions of the job's first n primaries
+        (GaussianBeam::createElectron as run_trajectory draws it)."""
+        centers = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1, 3)
+        job = JobDesc(1.0, beam_width_m, centers.shape[0], per_pixel, dptr(centers), seed, 1, 0, 50.0,
+                      primary_offset)
+        out = np.zeros((n, 4))
+        if lib().oracle_probe_primaries(ctypes.c_void_p(self.h), ctypes.byref(job), int(n), dptr(out)):
+            raise ValueError("bad primaries probe arguments")
+        return out[:, :3], out[:, 3].astype(int)
 
     # ---- probes (float64 in, float64 out; eV / nm) ----
     def rates(self, msm, energies_eV, n_mech):

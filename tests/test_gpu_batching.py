@@ -125,3 +125,69 @@ def test_many_small_jobs_of_uneven_size(M):
     for r, (mat, edge, e, n, seed) in zip(res, specs):
         assert r["stats"]["primaries"] == n and r["stats"]["queue_overflow"] == 0
         assert 0 < r["back"][0] + r["fwd"][0]
+
+
+def test_dead_items_do_not_end_a_launch(M):
+    """A launch whose items include long runs of electrons that are dead when
+    claimed -- primaries of a job whose beam starts outside the chamber, and
+    the queued electrons of a job stopped by the generation limit -- still
+    claims every item: a healthy job in the same launches matches its run
+    alone bit for bit, the dead job reports nothing, and the stopped job
+    counts its dropped electrons."""
+    from cudamonsel_amd import scenes
+    from cudamonsel_amd.abi import Engine
+    pm = scenes.flat("PMMA", beam_eV=500.0)
+    cu = scenes.flat("Cu", beam_eV=5000.0)
+    tuning = dict(max_generations=8, primaries_per_launch=3000, admission_order=1)
+    healthy = ([[0.0, 0.0, pm.beam_z]] * 4, 150, 500.0, 31)
+    with Engine(**tuning) as eng:
+        s_pm, s_cu = eng.upload(pm.monte.compileScene()), eng.upload(cu.monte.compileScene())
+        alone = eng.result(eng.submit(s_pm, healthy[0], healthy[1], healthy[2], 0.5e-9, healthy[3]))
+        outside = eng.submit(s_pm, [[0.0, 0.0, -0.5]] * 8, 20000, 500.0, 0.5e-9, 32)   # 0.5 m: beyond the chamber
+        stopped = eng.submit(s_cu, [[0.0, 0.0, cu.beam_z]], 60000, 5000.0, 0.5e-9, 33)
+        mixed = eng.submit(s_pm, healthy[0], healthy[1], healthy[2], 0.5e-9, healthy[3])
+        r_out, r_stop, r_mix = eng.result(outside), eng.result(stopped), eng.result(mixed)
+    _same(alone, r_mix)
+    assert alone["stats"]["queue_overflow"] == 0 and alone["stats"]["steps"] > 0
+    assert r_out["stats"]["primaries"] == 160000 and r_out["stats"]["steps"] == 0
+    assert int(r_out["back"].sum() + r_out["fwd"].sum()) == 0
+    assert r_stop["stats"]["queue_overflow"] > 0
+
+
+def test_cancel(M):
+    """monsel_job_cancel: a queued job ends at once; a running one ends after
+    its queued electrons are dropped, freeing its slot; jobs sharing its
+    launches, and jobs after it, match their runs alone."""
+    import time
+    from cudamonsel_amd import scenes
+    from cudamonsel_amd.abi import Engine
+    sc = scenes.line_on_substrate()
+    cen = scenes.scan_centers(np.linspace(-20, 20, 64), [0.0], sc.beam_z)
+    small = ([[0.0, 0.0, sc.beam_z]] * 4, 200, 500.0, 0.5e-9, 41)
+    with Engine() as eng:
+        sid = eng.upload(sc.monte.compileScene())
+        alone = eng.result(eng.submit(sid, *small))
+        # running: a 2.56 M-primary scan cancelled while a small job shares its launches
+        big = eng.submit(sid, cen, 40000, 500.0, 0.5e-9, 42)
+        mix = eng.submit(sid, *small)
+        time.sleep(0.02)
+        eng.cancel(big)
+        assert eng.wait_status(big, timeout_s=60) in (-8, 0)
+        _same(alone, eng.result(mix))
+        eng.release(big)
+        # after: the engine runs on with the freed slot and reservation
+        _same(alone, eng.result(eng.submit(sid, *small)))
+    with Engine(max_jobs_in_flight=1) as eng:
+        sid = eng.upload(sc.monte.compileScene())
+        first = eng.submit(sid, cen, 3000, 500.0, 0.5e-9, 43)
+        queued = eng.submit(sid, *small)
+        eng.cancel(queued)
+        assert eng.wait_status(queued, timeout_s=60) == -8
+        eng.release(queued)
+        r = eng.result(first)
+        assert r["stats"]["primaries"] == 64 * 3000 and r["stats"]["queue_overflow"] == 0
+        # cancelling a completed job changes nothing
+        done = eng.submit(sid, *small)
+        assert eng.wait_status(done, timeout_s=60) == 0
+        eng.cancel(done)
+        _same(alone, eng.result(done))

@@ -30,7 +30,8 @@ def si(oracle):
 
 
 def test_rates(pmma, si):
-    E = np.geomspace(0.06, 19000.0, 400).astype(np.float32).astype(np.float64)
+    # through the screened-Rutherford branch of the NIST Mott elements (>= 20 keV)
+    E = np.geomspace(0.06, 50000.0, 500).astype(np.float32).astype(np.float64)
     for sc, cs, pr, osc in (pmma, si):
         g = pr.rates(1, E, 3).astype(np.float64)
         o = osc.rates(1, E, 3)
@@ -54,6 +55,19 @@ def test_elastic_angles(pmma, slot):
     rng = np.random.default_rng(2 + slot)
     n = 20000
     E = np.geomspace(1.0, 19500.0, n).astype(np.float32).astype(np.float64)
+    u = rng.random((n, 2)).astype(np.float32).astype(np.float64)
+    g = pr.elastic_angle(1, slot, E, u).astype(np.float64)
+    o = osc.elastic_angle(1, slot, E, u)
+    assert np.max(np.abs(np.cos(g) - np.cos(o))) < 5e-5
+
+
+@pytest.mark.parametrize("slot", [0, 1, 2])
+def test_elastic_angles_screened_rutherford(pmma, slot):
+    """NISTMottRS above its tables (>= 20 keV, NISTMottRS.cu:151-153)."""
+    sc, cs, pr, osc = pmma
+    rng = np.random.default_rng(12 + slot)
+    n = 20000
+    E = np.geomspace(20100.0, 50000.0, n).astype(np.float32).astype(np.float64)
     u = rng.random((n, 2)).astype(np.float32).astype(np.float64)
     g = pr.elastic_angle(1, slot, E, u).astype(np.float64)
     o = osc.elastic_angle(1, slot, E, u)
@@ -220,14 +234,18 @@ def test_scatter_si_phonon(si):
     _compare_scatter(pr.scatter(1, 2, E, E, d, u), osc.scatter(1, 2, E, E, d, u))
 
 
-@pytest.mark.parametrize("method", [1, 2, 3])
-def test_scatter_tabulated(oracle, M, tmp_path, method):
+@pytest.mark.parametrize("method,dispersion", [(1, False), (2, False), (3, False), (1, True), (3, True)],
+                         ids=["1", "2", "3", "1-E0-dispersion", "3-E0-dispersion"])
+def test_scatter_tabulated(oracle, M, tmp_path, method, dispersion):
+    """TabulatedInelasticSM::scatter, with setE0fromDispersion(true) in the
+    last two cases (the core-level choice then compares the core energies
+    with the plasmon energy of TabulatedInelasticSM.cu:103-151)."""
     from cudamonsel_amd import scenes, tables
     from cudamonsel_amd.abi import probe_for
     paths = tables.synthetic_inelastic_tables(str(tmp_path))
     sc = scenes.Scene()
     ch = scenes._chamber(sc)
-    msm = scenes.si_tabulated(sc, paths, method_se=method)
+    msm = scenes.si_tabulated(sc, paths, method_se=method, e0_from_dispersion=dispersion)
     shp = scenes._half_space(0.0)
     reg = M.Region(ch, msm, shp)
     sc.hold(shp, reg)
