@@ -219,7 +219,10 @@ MD float rutherford_sigma(const MONSEL_CS DElem& el, float EeV) {
 MD float rutherford_cos(const MONSEL_CS DElem& el, float EeV, float r) {
   const float K = 5.44968e-19f / 1.60217653e-19f;
   float alpha = K * el.rz23 / EeV;
-  return 1.f - 2.0f * alpha * r / (1.f + alpha - r);
+  /* alpha - r + 1 summed as (1 - r) + alpha: 1 - r is exact for r >= 1/2,
+   * so the denominator keeps its relative precision when alpha and 1 - r are
+   * both small (E >= 20 keV, r near 1) */
+  return 1.f - 2.0f * alpha * r / ((1.f - r) + alpha);
 }
 
 /* NISTMottRS::totalCrossSection (NISTMottRS.cu:103-119), nm^2 */

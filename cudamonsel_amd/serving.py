@@ -208,10 +208,12 @@ def per_pixel_plain(engine, requests, sample=16, log=None):
                 images_per_hour=3600.0 * len(requests) / wall, pixels_sampled_per_request=sample)
 
 
-def serve(engine, requests, plain=False, hbm_budget=64 << 30, time_scale=1.0, poll_s=0.0005, log=None):
+def serve(engine, requests, plain=False, hbm_budget=64 << 30, time_scale=1.0, poll_s=0.0005, log=None,
+          keep_results=False):
     """Replays the request stream in real time (arrival times divided by
     time_scale) and returns per-request latencies and totals; log(msg), if
-    given, hears of every completed request."""
+    given, hears of every completed request.  With keep_results the
+    returned dict also maps every request id to its tallies ("results")."""
     cache = SceneCache(engine)
     for r in requests:
         _request_scene(cache, r)
@@ -221,6 +223,8 @@ def serve(engine, requests, plain=False, hbm_budget=64 << 30, time_scale=1.0, po
     running = {}          # job id -> (request, footprint)
     adm = Admission(hbm_budget)
     lat = {}
+    kept = {}
+    peak = 0
     t0 = time.perf_counter()
     done = 0
     while done < len(requests):
@@ -239,10 +243,13 @@ def serve(engine, requests, plain=False, hbm_budget=64 << 30, time_scale=1.0, po
             jid = engine.submit(sid, cen, r.per_pixel, r.beam_eV, 0.5e-9, 5000 + r.rid)
             running[jid] = (r, need)
             adm.admit(need)
+            peak = max(peak, len(running))
         finished = [j for j in running if engine.poll(j)]
         for j in finished:
             r, need = running.pop(j)
-            engine.result(j)
+            res = engine.result(j)
+            if keep_results:
+                kept[r.rid] = res
             adm.release(need)
             lat[r.rid] = (time.perf_counter() - t0) * time_scale - r.arrival
             done += 1
@@ -257,6 +264,9 @@ def serve(engine, requests, plain=False, hbm_budget=64 << 30, time_scale=1.0, po
     wall = time.perf_counter() - t0
     vals = [lat[r.rid] for r in requests]
     prim = sum(r.primaries for r in requests)
-    return dict(requests=len(requests), wall_s=wall, p50_s=percentile(vals, 50), p99_s=percentile(vals, 99),
-                images_per_hour=3600.0 * len(requests) / wall, primaries=prim, primaries_per_s=prim / wall,
-                latencies=vals)
+    out = dict(requests=len(requests), wall_s=wall, p50_s=percentile(vals, 50), p99_s=percentile(vals, 99),
+               images_per_hour=3600.0 * len(requests) / wall, primaries=prim, primaries_per_s=prim / wall,
+               latencies=vals, max_in_flight=peak)
+    if keep_results:
+        out["results"] = kept
+    return out

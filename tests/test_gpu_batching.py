@@ -138,7 +138,10 @@ def test_dead_items_do_not_end_a_launch(M):
     from cudamonsel_amd.abi import Engine
     pm = scenes.flat("PMMA", beam_eV=500.0)
     cu = scenes.flat("Cu", beam_eV=5000.0)
-    tuning = dict(max_generations=8, primaries_per_launch=3000, admission_order=1)
+    # the Cu job needs 20 launches to admit its primaries (3,000 per launch)
+    # and is stopped after 16; the healthy job, admitted first, completes in
+    # a few launches alone and mixed
+    tuning = dict(max_generations=16, primaries_per_launch=3000, admission_order=1)
     healthy = ([[0.0, 0.0, pm.beam_z]] * 4, 150, 500.0, 31)
     with Engine(**tuning) as eng:
         s_pm, s_cu = eng.upload(pm.monte.compileScene()), eng.upload(cu.monte.compileScene())
@@ -147,6 +150,7 @@ def test_dead_items_do_not_end_a_launch(M):
         stopped = eng.submit(s_cu, [[0.0, 0.0, cu.beam_z]], 60000, 5000.0, 0.5e-9, 33)
         mixed = eng.submit(s_pm, healthy[0], healthy[1], healthy[2], 0.5e-9, healthy[3])
         r_out, r_stop, r_mix = eng.result(outside), eng.result(stopped), eng.result(mixed)
+    assert r_mix["stats"]["queue_overflow"] == 0, (alone["stats"], r_mix["stats"])
     _same(alone, r_mix)
     assert alone["stats"]["queue_overflow"] == 0 and alone["stats"]["steps"] > 0
     assert r_out["stats"]["primaries"] == 160000 and r_out["stats"]["steps"] == 0
