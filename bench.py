@@ -59,6 +59,37 @@ def profile_figures():
     return {k: float(v) for k, v in (kv.split("=") for kv in m.group(1).split())}
 
 
+def roofline_block(achieved, prof, k_ms, launches):
+    """Roofline of k_transport.  The fraction is algorithmic bytes per
+    second over HBM peak (MFMA has no part in this path).  The measured
+    figures say what limits the kernel: its HBM traffic is a small fraction
+    of peak, and of the VALU lane-slots it issues, valu_busy x
+    lane_utilisation do useful work, so it is bound by instruction issue and
+    lane divergence, not by HBM."""
+    ms_launch = k_ms / max(launches, 1)
+    traffic = int(prof["traffic_bytes_per_launch"]) if "traffic_bytes_per_launch" in prof else None
+    vb, lu = prof.get("valu_busy"), prof.get("lane_utilisation")
+    out = {
+        "bound": "hbm",
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "traffic_source": "profiles/k_transport_linescan.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+        "measured_traffic_frac": (round(traffic / (ms_launch * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                                  if traffic and ms_launch > 0 else None),
+        "limiter": "VALU issue and lane divergence (measured HBM traffic is far below peak)",
+        "valu_busy": vb,
+        "lane_utilisation": lu,
+        "valu_frac": round(vb * lu, 4) if vb is not None and lu is not None else None,
+        "kernel": "k_transport",
+        "kernel_ms_per_launch": round(ms_launch, 3),
+        "bytes_per_step": BYTES_PER_STEP,
+    }
+    return out
+
+
 def log(msg):
     """Progress on stderr (stdout carries only the JSON line)."""
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -79,14 +110,34 @@ def copy_calibration(nbytes):
     hip.hipFree(b)
 
 
+def host_cpu():
+    """(logical CPUs the OS shows, CPU model name)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return os.cpu_count() or 1, model
+
+
+# host cores that belong to one GPU of the node (the worker pools of a
+# one-GPU job are sized to this share, whatever the OS shows)
+HOST_CORES_PER_GPU = 16
+
+
 def cpu_baseline(sc, centers, target_s=10.0):
     """Time the oracle on a bounded sample of the scan (all 128 pixels, a few
-    primaries each), sized to about target_s seconds of wall time."""
+    primaries each), sized to about target_s seconds of wall time, with one
+    worker per host core of this GPU's share."""
     import cudamonsel_amd as M
     from oracle import oracle as O
     cs = sc.monte.compileScene()
     osc = O.OracleScene(cs)
-    threads = min(os.cpu_count() or 1, 16)
+    nproc, model = host_cpu()
+    threads = min(nproc, HOST_CORES_PER_GPU)
     per = 4
     t0 = time.time()
     osc.run(centers, per, M.ToSI.eV(500.0), 0.5e-9, seed=5, nbins=50, threads=threads)
@@ -98,7 +149,9 @@ def cpu_baseline(sc, centers, target_s=10.0):
     n = per * centers.shape[0]
     return {"value": round(n / dt, 1), "unit": "primaries/s", "cores": threads, "kind": "port",
             "sample": f"line scan, 128 pixels x {per} primaries ({n} primaries, {o['stats']['steps']} steps) "
-                      f"in {dt:.1f} s on the FP64 oracle"}
+                      f"in {dt:.1f} s on the FP64 oracle",
+            "host_logical_cpus": nproc, "cpu_model": model,
+            "cores_note": f"one worker per core of the {HOST_CORES_PER_GPU}-core share of one GPU"}
 
 
 def batched_leg(n_jobs, tuning):
@@ -127,32 +180,57 @@ def batched_leg(n_jobs, tuning):
             "results_identical": bool(same)}
 
 
-def served_leg(duration_s, rate, tuning):
-    """The serving configuration on a shortened stream: Poisson arrivals at
-    `rate`/s for duration_s s of SEM-image requests (side 32-256 px, 50-500
-    primaries/pixel, 0.3-2 keV, random PMMA line arrays or contact holes on
-    Si), continuous batching against one request at a time, both on the
-    same engine tuning."""
+def _stream_legs(reqs, duration_s, tuning):
+    """Served (continuous batching) and plain (one request at a time) runs of
+    one request stream, with offered and achieved load."""
     from cudamonsel_amd import serving
     from cudamonsel_amd.abi import Engine
-    reqs = serving.request_stream(rate_per_s=rate, duration_s=duration_s, seed=5)
-    out = {"stream": f"Poisson {rate}/s for {duration_s:g} s, {len(reqs)} requests, "
-                     f"{sum(r.primaries for r in reqs)} primaries"}
-    out["tuning"] = tuning
-    log(f"served leg: {out['stream']}, tuning {tuning}")
+    prim = sum(r.primaries for r in reqs)
+    out = {"stream": f"{len(reqs)} requests over {duration_s:g} s, {prim} primaries",
+           "offered_primaries_per_s": round(prim / duration_s, 1)}
     for name, plain in (("served", False), ("plain", True)):
         # served: the request with the fewest primaries left gets new launches first
         with Engine(**dict(tuning, admission_order=0 if plain else 1)) as eng:
             r = serving.serve(eng, reqs, plain=plain, log=log)
         log(f"{name}: {r['wall_s']:.2f} s, p50 {r['p50_s']:.2f} s, p99 {r['p99_s']:.2f} s")
         out[name] = {k: round(v, 3) if isinstance(v, float) else v for k, v in r.items() if k != "latencies"}
+        out[name]["achieved_primaries_per_s"] = out[name].pop("primaries_per_s")
     out["images_per_hour_ratio"] = round(out["served"]["images_per_hour"] / out["plain"]["images_per_hour"], 3)
+    return out
+
+
+def served_leg(duration_s, rate, tuning, light_rate, light_s):
+    """The serving configuration on shortened streams of SEM-image requests
+    (side 32-256 px, 50-500 primaries/pixel, 0.3-2 keV, random PMMA line
+    arrays or contact holes on Si), continuous batching against one request
+    at a time on the same engine tuning: at the configuration's arrival rate
+    (whose offered load exceeds what one GPU transports, so both forms drain
+    a growing queue), and at a rate below capacity (where p99 is a service
+    latency).  Beside them, the reference's call pattern -- one job per
+    pixel -- as an estimate from sampled pixels, checked against one small
+    request run in full."""
+    from cudamonsel_amd import serving
+    from cudamonsel_amd.abi import Engine
+    out = {"tuning": tuning}
+    reqs = serving.request_stream(rate_per_s=rate, duration_s=duration_s, seed=5)
+    log(f"served leg: Poisson {rate}/s for {duration_s:g} s, {len(reqs)} requests, tuning {tuning}")
+    out["at_config_rate"] = dict(_stream_legs(reqs, duration_s, tuning), arrival_rate_per_s=rate)
+    light = serving.request_stream(rate_per_s=light_rate, duration_s=light_s, seed=6)
+    log(f"served leg below capacity: Poisson {light_rate}/s for {light_s:g} s, {len(light)} requests")
+    out["below_capacity"] = dict(_stream_legs(light, light_s, tuning), arrival_rate_per_s=light_rate)
     # the reference's call pattern (one job per pixel), timed on a sample of pixels per request
     with Engine() as eng:
         r = serving.per_pixel_plain(eng, reqs, sample=16, log=None)
-    log(f"per-pixel plain (estimated): {r['wall_s']:.2f} s, p50 {r['p50_s']:.2f} s, p99 {r['p99_s']:.2f} s")
-    out["plain_per_pixel"] = {k: round(v, 3) if isinstance(v, float) else v for k, v in r.items()}
-    out["images_per_hour_ratio_vs_per_pixel"] = round(out["served"]["images_per_hour"] / r["images_per_hour"], 2)
+        chk = serving.per_pixel_check(eng, reqs, sample=16)
+    log(f"per-pixel plain (estimated): {r['wall_s']:.2f} s, p50 {r['p50_s']:.2f} s, p99 {r['p99_s']:.2f} s; "
+        f"check: request {chk['request']} estimated {chk['estimated_s']:.2f} s, run {chk['measured_s']:.2f} s")
+    est = {k: round(v, 3) if isinstance(v, float) else v for k, v in r.items()}
+    est["method"] = ("estimate: 16 pixels of every request run as one job each, service time = mean pixel "
+                     "time x pixels, the stream replayed one request at a time on those service times")
+    est["check_one_request_in_full"] = {k: round(v, 3) if isinstance(v, float) else v for k, v in chk.items()}
+    out["plain_per_pixel_estimate"] = est
+    out["images_per_hour_ratio_vs_per_pixel_estimate"] = round(
+        out["at_config_rate"]["served"]["images_per_hour"] / r["images_per_hour"], 2)
     return out
 
 
@@ -172,6 +250,9 @@ def main():
     ap.add_argument("--batch-jobs", type=int, default=512, help="jobs of the batched leg (0: skip)")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="arrival window of the served leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=2.0)
+    ap.add_argument("--serve-light-rate", type=float, default=0.4,
+                    help="arrival rate of the below-capacity served stream (req/s)")
+    ap.add_argument("--serve-light-seconds", type=float, default=25.0)
     ap.add_argument("--serve-tuning", default='{"geometry_batch": 8}',
                     help="JSON object of engine tuning fields for the served leg, applied to the served and "
                          "the one-at-a-time engine alike (measured: +5 %% images/h and a lower p99 than "
@@ -256,20 +337,7 @@ def main():
             "mean_BSE_yield": round(float(bse), 4),
             "ref_compat": "reference",
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(achieved, 2),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": int(prof["traffic_bytes_per_launch"]) if "traffic_bytes_per_launch" in prof else None,
-            "traffic_source": "profiles/k_transport_linescan.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
-            "valu_busy": prof.get("valu_busy"),
-            "lane_utilisation": prof.get("lane_utilisation"),
-            "kernel": "k_transport",
-            "kernel_ms_per_launch": round(k_ms / max(launches, 1), 3),
-            "bytes_per_step": BYTES_PER_STEP,
-        },
+        "roofline": roofline_block(achieved, prof, k_ms, launches),
     }
     if args.batch_jobs > 0:
         import json as _json
@@ -278,7 +346,8 @@ def main():
         out["batched"] = batched_leg(args.batch_jobs, tuning)
     if args.serve_seconds > 0:
         out["served"] = served_leg(args.serve_seconds, args.serve_rate,
-                                   json.loads(args.serve_tuning) if args.serve_tuning else {})
+                                   json.loads(args.serve_tuning) if args.serve_tuning else {},
+                                   args.serve_light_rate, args.serve_light_seconds)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc, centers)
     print(json.dumps(out), flush=True)

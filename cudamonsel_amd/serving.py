@@ -208,6 +208,28 @@ def per_pixel_plain(engine, requests, sample=16, log=None):
                 images_per_hour=3600.0 * len(requests) / wall, pixels_sampled_per_request=sample)
 
 
+def per_pixel_check(engine, requests, sample=16):
+    """Checks the per-pixel estimate on the smallest request of the stream:
+    its estimate (sample pixels timed, mean x pixels) against all its pixels
+    run one job each."""
+    cache = SceneCache(engine)
+    r = min(requests, key=lambda q: (q.side * q.side * q.per_pixel, q.rid))
+    sc, cs, sid = _request_scene(cache, r)
+    cen = raster_centers(r.side, sc.beam_z)
+    engine.result(engine.submit(sid, cen[:1], r.per_pixel, r.beam_eV, 0.5e-9, 5000 + r.rid))   # warm-up
+    idx = np.linspace(0, cen.shape[0] - 1, min(sample, cen.shape[0])).astype(int)
+    t0 = time.perf_counter()
+    for k in idx:
+        engine.result(engine.submit(sid, cen[k:k + 1], r.per_pixel, r.beam_eV, 0.5e-9, 5000 + r.rid))
+    est = (time.perf_counter() - t0) / len(idx) * cen.shape[0]
+    t0 = time.perf_counter()
+    for k in range(cen.shape[0]):
+        engine.result(engine.submit(sid, cen[k:k + 1], r.per_pixel, r.beam_eV, 0.5e-9, 5000 + r.rid))
+    run = time.perf_counter() - t0
+    return dict(request=r.rid, side=r.side, per_pixel=r.per_pixel, estimated_s=est, measured_s=run,
+                ratio=est / run)
+
+
 def serve(engine, requests, plain=False, hbm_budget=64 << 30, time_scale=1.0, poll_s=0.0005, log=None,
           keep_results=False):
     """Replays the request stream in real time (arrival times divided by
