@@ -37,3 +37,7 @@ g = dict(source="CudaMONSEL/CudaMONSEL/outputs/output_release.txt",
                             "columns": prof},
          elapsed_s=float(lines[-1]))
 json.dump(g, open(__file__.replace("make_reference_run_log.py", "reference_run_log.json"), "w"), indent=1)
+# the head of the run log (header, column line, first rows) and its last
+# line (elapsed seconds), for the parser of the five-column text
+with open(__file__.replace("make_reference_run_log.py", "reference_output_excerpt.txt"), "w") as f:
+    f.write("\n".join(lines[:30] + lines[-1:]))
