@@ -59,7 +59,7 @@ SYMBOLS = [
     "monsel_job_release", "monsel_run", "monsel_probe_rates", "monsel_probe_elastic_angle", "monsel_probe_csd",
     "monsel_probe_find_end_of_step", "monsel_probe_locate", "monsel_probe_nu_table", "monsel_probe_scatter",
     "monsel_probe_barrier", "monsel_probe_safety", "monsel_job_cancel", "monsel_probe_philox",
-    "monsel_probe_hist_bin", "monsel_probe_primaries",
+    "monsel_probe_hist_bin", "monsel_probe_primaries", "monsel_run_static",
 ]
 
 
@@ -277,11 +277,25 @@ class Engine:
         _check(self.L.monsel_job_footprint(self.ctx, ctypes.byref(jd), ctypes.byref(nb)), "monsel_job_footprint")
         return nb.value
 
-    def result(self, job_id, timeout_s=None):
-        """Waits for the job, returns its tallies and statistics, releases it."""
-        us = -1 if timeout_s is None else int(timeout_s * 1e6)
-        _check(self.L.monsel_job_wait(self.ctx, ctypes.c_uint64(job_id), ctypes.c_int64(us)), "monsel_job_wait")
-        n, nb = self._keep.pop(job_id)
+    def run(self, scene_id, centers_m, per_pixel, beam_eV, sigma_m, seed, nbins=50, se_cutoff_eV=50.0,
+            max_steps=0, static=False):
+        """One job through monsel_run, or with static=True through
+        monsel_run_static (no device allocation); returns its tallies and
+        statistics as result() does."""
+        c = np.ascontiguousarray(centers_m, dtype=np.float64).reshape(-1, 3)
+        n = c.shape[0]
+        jd = JobDesc(beam_eV * 1.60217653e-19, sigma_m, n, int(per_pixel), c.ctypes.data_as(c_double_p),
+                     int(seed), int(nbins), int(max_steps), float(se_cutoff_eV), 0)
+        r, t = self._tally_buffers(n, int(nbins))
+        st = JobStats()
+        fn = self.L.monsel_run_static if static else self.L.monsel_run
+        _check(fn(self.ctx, ctypes.c_uint64(scene_id), ctypes.byref(jd), ctypes.byref(t), ctypes.byref(st)),
+               "monsel_run_static" if static else "monsel_run")
+        r["stats"] = {f: getattr(st, f) for f, _ in JobStats._fields_}
+        return r
+
+    @staticmethod
+    def _tally_buffers(n, nb):
         r = {k: np.zeros(n, np.uint64) for k in ("back", "fwd", "se")}
         r.update({k: np.zeros(n) for k in ("se_sq", "bse_sq", "back_energy")})
         r["back_energy_hist"] = np.zeros(nb + 2, np.uint64)
@@ -293,6 +307,14 @@ class Engine:
             setattr(t, k, r[k].ctypes.data_as(c_u64_p))
         for k in ("se_sq", "bse_sq", "back_energy"):
             setattr(t, k, r[k].ctypes.data_as(c_double_p))
+        return r, t
+
+    def result(self, job_id, timeout_s=None):
+        """Waits for the job, returns its tallies and statistics, releases it."""
+        us = -1 if timeout_s is None else int(timeout_s * 1e6)
+        _check(self.L.monsel_job_wait(self.ctx, ctypes.c_uint64(job_id), ctypes.c_int64(us)), "monsel_job_wait")
+        n, nb = self._keep.pop(job_id)
+        r, t = self._tally_buffers(n, nb)
         _check(self.L.monsel_job_read(self.ctx, ctypes.c_uint64(job_id), ctypes.byref(t)), "monsel_job_read")
         st = JobStats()
         _check(self.L.monsel_job_stats_get(self.ctx, ctypes.c_uint64(job_id), ctypes.byref(st)), "monsel_job_stats_get")

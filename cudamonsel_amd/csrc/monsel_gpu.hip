@@ -1177,6 +1177,22 @@ __global__ void k_reduce_primaries(const JobRec* jobs, int slot, uint32_t n, uns
   }
 }
 
+/* Zeroes n u64 words (clears device buffers without hipMemsetAsync). */
+__global__ void k_zero_u64(unsigned long long* p, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0ull;
+}
+
+/* Per-pixel sums of squares and fixed-point energies (u64, pixel tallies 3..5
+ * of k_reduce_primaries) as the doubles of monsel_tallies: se_sq, bse_sq and
+ * back_energy (eV), each [np]. */
+__global__ void k_pixels_to_double(const unsigned long long* px, int np, double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 3 * np) return;
+  const double v = (double)px[3 * (size_t)np + i];
+  out[i] = i >= 2 * np ? v / kEnergyScale : v;
+}
+
 /* ---- probe kernels ---------------------------------------------------- */
 
 __global__ void k_probe_rates(DevScene S, int msm, int n, const float* E, float* out) {
@@ -1931,6 +1947,43 @@ HostScene merge_scenes(const std::vector<const HostScene*>& parts, std::vector<i
 }
 
 /* ===================================================================== */
+/* allocation-free jobs: one static device array per GPU                  */
+/* ===================================================================== */
+
+/* Every device buffer of monsel_run_static beyond the caller's inputs and
+ * outputs (job record, launch counters, per-primary and per-pixel tallies,
+ * histograms, pixel centres, both generation queues and the escape records)
+ * is carved out of this array.  Each GPU the library runs on holds its own
+ * copy from the code object's load until the process ends.  Host code never
+ * passes `scratch` itself (on the host it is a stand-in, not a device
+ * address): it uses the address hipGetSymbolAddress gives per device. */
+constexpr size_t kStaticScratchBytes = (size_t)128 << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+constexpr int kMaxStaticDevices = 64;
+static std::mutex g_static_mu[kMaxStaticDevices]; /* one call at a time per GPU */
+static void* g_static_ptr[kMaxStaticDevices];     /* device address of `scratch` per GPU */
+static hipError_t g_static_status[kMaxStaticDevices]; /* of the lookup, when it failed */
+static std::mutex g_static_init_mu;
+
+/* Resolves the device address of `scratch` on `device`, which must be the
+ * current device (monsel_create calls this once per context).  A failed
+ * lookup is kept and reported by monsel_run_static; the rest of the context
+ * does not need the array. */
+static void static_scratch_resolve(int device) {
+  if (device < 0 || device >= kMaxStaticDevices) return; /* monsel_run_static reports it */
+  std::lock_guard<std::mutex> lk(g_static_init_mu);
+  if (g_static_ptr[device]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    g_static_status[device] = e != hipSuccess ? e : hipErrorInvalidSymbol;
+    (void)hipGetLastError(); /* the failure belongs to monsel_run_static, not to later calls */
+    return;
+  }
+  g_static_ptr[device] = p;
+}
+
+/* ===================================================================== */
 /* host: jobs                                                             */
 /* ===================================================================== */
 
@@ -2517,6 +2570,7 @@ int monsel_create(int device_ordinal, const monsel_tuning* tuning, monsel_ctx* o
     C->device = device_ordinal;
     if (tuning) C->tuning = *tuning;
     HIP_TRY(hipSetDevice(device_ordinal));
+    static_scratch_resolve(device_ordinal);
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, device_ordinal));
     C->n_cu = prop.multiProcessorCount;
@@ -2792,6 +2846,228 @@ int monsel_run(monsel_ctx ctx, monsel_scene_id scene, const monsel_job_desc* job
   monsel_job_release(ctx, id);
   g_err = keep;
   return st;
+}
+
+/* Allocation-free variant of monsel_run: the same job on the same kernels,
+ * run on the calling thread, with every device buffer beyond the caller's
+ * inputs and outputs carved out of the static array `scratch`. */
+int monsel_run_static(monsel_ctx ctx, monsel_scene_id scene, const monsel_job_desc* job, monsel_tallies* out,
+                      monsel_job_stats* stats) {
+  if (!ctx) return set_err(MONSEL_E_INVALID, "NULL context");
+  const int v = validate_job(job);
+  if (v) return v;
+  if (ctx->device >= kMaxStaticDevices)
+    return set_err(MONSEL_E_UNSUPPORTED, "monsel_run_static: device ordinal above " +
+                                             std::to_string(kMaxStaticDevices - 1));
+  std::shared_ptr<SceneDev> sd = find_scene(ctx, scene);
+  if (!sd) return set_err(MONSEL_E_NOTFOUND, "unknown scene");
+  return guarded([&] {
+    const auto t0 = std::chrono::steady_clock::now();
+    std::lock_guard<std::mutex> lk(g_static_mu[ctx->device]);
+    char* const base = (char*)g_static_ptr[ctx->device];
+    if (!base)
+      throw MonselError(MONSEL_E_DEVICE, std::string("monsel_run_static: no device address for the static buffer: ") +
+                                             hipGetErrorString(g_static_status[ctx->device]));
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t stream = ctx->stream;
+    const monsel_job_desc& jd = *job;
+    const int np = jd.n_pixels;
+    const int nbins = jd.n_energy_bins > 0 ? jd.n_energy_bins : 1;
+    const double E0eV = jd.beam_energy / kE;
+    const uint64_t total = (uint64_t)np * (uint64_t)jd.primaries_per_pixel;
+    const uint32_t K = (uint32_t)std::min<double>(4096., 16. + E0eV * sd->host.secondaries_per_eV);
+    /* layout: fixed-size records, then the job's tallies (cleared), then the
+     * pixel centres, then the queues take what is left */
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+      const size_t o = off;
+      off += (bytes + 255) & ~(size_t)255;
+      return o;
+    };
+    const size_t o_rec = take(sizeof(JobRec)), o_cnt = take(8 * 4), o_seg = take(4 * 4);
+    const size_t o_zero = off;
+    const size_t o_st = take(8 * 8 * kStatStripes), o_hist = take((size_t)hist_len(nbins) * 8);
+    const size_t o_px = take((size_t)np * 6 * 8);
+    const size_t o_pc = take(total * 4), o_pf = take(total * 4), o_pe = take(total * 8);
+    const size_t zero_end = off;
+    const size_t o_dbl = take((size_t)np * 3 * 8), o_cen = take((size_t)np * 3 * 8);
+    const size_t per_entry = 2 * kQueueEntryBytes + kEscEntryBytes;
+    const uint64_t cap = off + 3 * 256 < kStaticScratchBytes
+                             ? std::min<uint64_t>((kStaticScratchBytes - off - 3 * 256) / per_entry, kMaxQueueCapacity)
+                             : 0;
+    if (cap < 4096 || cap < K)
+      throw MonselError(MONSEL_E_TOOLARGE,
+                        "monsel_run_static: " + std::to_string(np) + " pixels x " +
+                            std::to_string(jd.primaries_per_pixel) + " primaries need " + std::to_string(off) +
+                            " bytes of tallies, leaving room for " + std::to_string(cap) +
+                            " queued electrons (at least " + std::to_string(std::max<uint32_t>(K, 4096)) +
+                            " needed) in its " + std::to_string(kStaticScratchBytes >> 20) + " MiB static buffer");
+    const size_t o_q0 = take(cap * kQueueEntryBytes), o_q1 = take(cap * kQueueEntryBytes);
+    const size_t o_esc = take(cap * kEscEntryBytes);
+    if (off > kStaticScratchBytes) throw MonselError(MONSEL_E_DEVICE, "monsel_run_static: layout overflow");
+    JobRec* d_rec = (JobRec*)(base + o_rec);
+    unsigned int* d_cnt = (unsigned int*)(base + o_cnt);
+    uint32_t* d_seg = (uint32_t*)(base + o_seg);
+    unsigned long long* d_st = (unsigned long long*)(base + o_st);
+    unsigned long long* d_hist = (unsigned long long*)(base + o_hist);
+    unsigned long long* d_px = (unsigned long long*)(base + o_px);
+    double* d_dbl = (double*)(base + o_dbl);
+    float* d_q[2] = {(float*)(base + o_q0), (float*)(base + o_q1)};
+    float* d_esc = (float*)(base + o_esc);
+    /* the previous call's contents stay in the array: clear the tallies */
+    const size_t nz = (zero_end - o_zero) / 8;
+    g_hip_call = "launch k_zero_u64";
+    hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)std::min<size_t>((nz + 255) / 256, (size_t)ctx->n_cu * 8)),
+                       dim3(256), 0, stream, (unsigned long long*)(base + o_zero), nz);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(base + o_cen, jd.pixel_center, (size_t)np * 3 * 8, hipMemcpyHostToDevice, stream));
+    const HostScene& H = sd->host;
+    JobRec R;
+    memset(&R, 0, sizeof(R));
+    R.lin0 = job_lineage(jd.seed);
+    R.E0 = (float)E0eV;
+    R.width = (float)(jd.beam_width * 1e9);
+    R.centers = (const double*)(base + o_cen);
+    R.n_pixels = np;
+    R.ppp = jd.primaries_per_pixel;
+    R.prim_base = (uint32_t)jd.primary_offset;
+    R.root = 0; /* the scene's own chamber region (the scene is not merged with others) */
+    R.max_steps = jd.max_steps > 0 ? jd.max_steps : 1000000;
+    R.nbins = nbins;
+    R.maxSEbin = (int)(jd.se_cutoff_ev / (E0eV / nbins));
+    R.E0eV = E0eV;
+    for (int k = 0; k < 3; ++k) R.chamber_c[k] = H.chamber_c[k];
+    R.chamber_r = H.chamber_r;
+    R.prim_counts = (uint32_t*)(base + o_pc);
+    R.prim_fwd = (uint32_t*)(base + o_pf);
+    R.prim_energy = (unsigned long long*)(base + o_pe);
+    R.hist = d_hist;
+    R.stats = d_st;
+    HIP_TRY(hipMemcpyAsync(d_rec, &R, sizeof(JobRec), hipMemcpyHostToDevice, stream));
+    /* generations, as run_generation does for one job (no queue sort) */
+    const monsel_tuning& T = ctx->tuning;
+    const int maxGen = T.max_generations > 0 ? T.max_generations : 100000;
+    const uint64_t perLaunch = T.primaries_per_launch > 0 ? (uint64_t)T.primaries_per_launch : cap;
+    const int block = T.block_size > 0 ? T.block_size : 256;
+    const int grid = ctx->n_cu * (ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : 4);
+    uint64_t launched = 0, reserved = 0;
+    uint32_t in_count = 0;
+    int cur = 0, generations = 0;
+    LdsJobs lj;
+    memset(&lj, 0, sizeof(lj));
+    if ((size_t)hist_len(nbins) * 4 <= 64 * 1024) {
+      lj.n = 1;
+      lj.off[1] = hist_len(nbins);
+    }
+    while (launched < total || in_count > 0) {
+      const uint64_t avail = cap > reserved ? cap - reserved : 0;
+      const uint64_t n = std::min<uint64_t>(std::min<uint64_t>(total - launched, avail / K), perLaunch);
+      if (n == 0 && in_count == 0) throw MonselError(MONSEL_E_DEVICE, "monsel_run_static: no work admitted");
+      const uint32_t segs[4] = {(uint32_t)n, 0u, (uint32_t)launched, 0u};
+      HIP_TRY(hipMemcpyAsync(d_seg, segs, sizeof(segs), hipMemcpyHostToDevice, stream));
+      g_hip_call = "launch k_zero_u64";
+      hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, stream, (unsigned long long*)d_cnt, (size_t)4);
+      HIP_TRY(hipGetLastError());
+      GenDev G;
+      memset(&G, 0, sizeof(G));
+      G.jobs = d_rec;
+      G.seg_end = d_seg;
+      G.seg_job = d_seg + 1;
+      G.seg_first = d_seg + 2;
+      G.n_seg = n > 0 ? 1 : 0;
+      G.n_prim = (uint32_t)n;
+      G.in_f = d_q[cur];
+      G.in_u = (const uint32_t*)(d_q[cur] + (size_t)kQF * cap);
+      G.in_cap = (uint32_t)cap;
+      G.in_n = in_count;
+      G.claim = d_cnt;
+      G.out_f = d_q[cur ^ 1];
+      G.out_u = (uint32_t*)(d_q[cur ^ 1] + (size_t)kQF * cap);
+      G.out_cap = (uint32_t)cap;
+      G.out_count = d_cnt + 1;
+      G.esc_f = d_esc;
+      G.esc_u = (uint32_t*)(d_esc + (size_t)kEF * cap);
+      G.esc_cap = (uint32_t)cap;
+      G.esc_count = d_cnt + 2;
+      G.steps_per_launch = T.steps_per_launch;
+      G.geo_batch = T.geometry_batch > 0 ? T.geometry_batch : kDefaultGeometryBatch;
+      G.handoff = T.spawn_handoff >= 0 ? 1 : 0;
+      G.tail_lanes = T.tail_requeue < 0 || n + in_count < kTailMinItems
+                         ? 0
+                         : (T.tail_requeue > 0 ? T.tail_requeue : kDefaultTailLanes);
+      g_hip_call = "launch k_transport";
+      if (sd->dev.has_tabulated)
+        hipLaunchKernelGGL(k_transport<true>, dim3(grid), dim3(block), 0, stream, sd->dev, G);
+      else
+        hipLaunchKernelGGL(k_transport<false>, dim3(grid), dim3(block), 0, stream, sd->dev, G);
+      HIP_TRY(hipGetLastError());
+      g_hip_call = "launch k_tally_escapes";
+      hipLaunchKernelGGL(k_tally_escapes, dim3(ctx->n_cu * 4), dim3(256), (size_t)lj.off[lj.n] * 4, stream, sd->dev,
+                         (const JobRec*)d_rec, (const float*)G.esc_f, (const uint32_t*)G.esc_u, (uint32_t)cap,
+                         (const unsigned int*)(d_cnt + 2), lj);
+      HIP_TRY(hipGetLastError());
+      unsigned int hc[4] = {0u, 0u, 0u, 0u};
+      HIP_TRY(hipMemcpyAsync(hc, d_cnt, sizeof(hc), hipMemcpyDeviceToHost, stream));
+      HIP_TRY(hipStreamSynchronize(stream));
+      if ((uint64_t)hc[0] < n + (uint64_t)in_count)
+        throw MonselError(MONSEL_E_DEVICE, "transport launch ended with unclaimed items");
+      in_count = hc[1] > cap ? (uint32_t)cap : hc[1];
+      cur ^= 1;
+      launched += n;
+      reserved = std::min<uint64_t>(reserved + n * K, (uint64_t)in_count * K);
+      if (++generations > maxGen && !R.killed) {
+        /* generation limit: the remaining electrons are dropped and counted */
+        R.killed = 1;
+        HIP_TRY(hipMemcpyAsync(&d_rec->killed, &R.killed, sizeof(int), hipMemcpyHostToDevice, stream));
+      }
+    }
+    if (total > 0) {
+      g_hip_call = "launch k_reduce_primaries";
+      hipLaunchKernelGGL(k_reduce_primaries, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream,
+                         (const JobRec*)d_rec, 0, (uint32_t)total, d_px);
+      HIP_TRY(hipGetLastError());
+    }
+    g_hip_call = "launch k_pixels_to_double";
+    hipLaunchKernelGGL(k_pixels_to_double, dim3((unsigned)((3 * np + 255) / 256)), dim3(256), 0, stream,
+                       (const unsigned long long*)d_px, np, d_dbl);
+    HIP_TRY(hipGetLastError());
+    unsigned long long sts[8 * kStatStripes];
+    HIP_TRY(hipMemcpyAsync(sts, d_st, sizeof(sts), hipMemcpyDeviceToHost, stream));
+    if (out) {
+      auto d2h = [&](void* dst, const void* src, size_t bytes) {
+        if (dst) HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
+      };
+      const size_t nb2 = (size_t)nbins + 2;
+      d2h(out->back, d_px, (size_t)np * 8);
+      d2h(out->fwd, d_px + np, (size_t)np * 8);
+      d2h(out->se, d_px + 2 * (size_t)np, (size_t)np * 8);
+      d2h(out->se_sq, d_dbl, (size_t)np * 8);
+      d2h(out->bse_sq, d_dbl + np, (size_t)np * 8);
+      d2h(out->back_energy, d_dbl + 2 * (size_t)np, (size_t)np * 8);
+      d2h(out->back_energy_hist, d_hist, nb2 * 8);
+      d2h(out->fwd_energy_hist, d_hist + nb2, nb2 * 8);
+      d2h(out->elevation_hist, d_hist + 2 * nb2, 182 * 8);
+      d2h(out->azimuth_hist, d_hist + 2 * nb2 + 182, 362 * 8);
+    }
+    HIP_TRY(hipStreamSynchronize(stream));
+    if (stats) {
+      unsigned long long st[8] = {0ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull};
+      for (int k = 0; k < 8 * kStatStripes; ++k) st[k % 8] += sts[k];
+      memset(stats, 0, sizeof(*stats));
+      stats->primaries = total;
+      stats->steps = st[kStatSteps];
+      stats->electrons = st[kStatElectrons];
+      stats->escapes = st[kStatEscapes];
+      stats->watchdog_drops = st[kStatWatchdog];
+      stats->queue_overflow = st[kStatOverflow];
+      stats->generations = generations;
+      stats->transport_launches = generations;
+      const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+      stats->device_ms = ms;
+      stats->total_ms = ms;
+    }
+    return MONSEL_OK;
+  });
 }
 
 }  /* extern "C" */

@@ -49,7 +49,8 @@ typedef enum monsel_status {
   MONSEL_E_NOTFOUND = -5,    /* unknown scene or job id */
   MONSEL_E_TIMEOUT = -6,     /* monsel_job_wait timed out */
   MONSEL_E_NOTREADY = -7,    /* job still running */
-  MONSEL_E_CANCELLED = -8    /* job ended by monsel_job_cancel: no results */
+  MONSEL_E_CANCELLED = -8,   /* job ended by monsel_job_cancel: no results */
+  MONSEL_E_TOOLARGE = -9     /* job exceeds the fixed device buffer of monsel_run_static */
 } monsel_status;
 
 /* Reference-compatibility switches ("RefCompat").  Each bit reproduces one
@@ -332,6 +333,24 @@ int monsel_job_cancel(monsel_ctx ctx, monsel_job_id job);
 /* Convenience: submit + wait + read + release. */
 int monsel_run(monsel_ctx ctx, monsel_scene_id scene, const monsel_job_desc* job,
                monsel_tallies* out, monsel_job_stats* stats);
+
+/* monsel_run without device allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (tight serving loops,
+ * code that forbids hipMalloc).  The same job on the same kernels, with the
+ * same results as monsel_run, run on the calling thread; every device buffer
+ * beyond the caller's inputs (job, pixel centres, the uploaded scene) and
+ * outputs (tallies, stats) lives in one static device array of 128 MiB per
+ * GPU, held from the library's load on that GPU until the process ends.
+ * The tallies of a job take 16 B per primary + 96 B per pixel + 8 B per
+ * histogram bin; the rest holds the generation queues (148 B per queued
+ * electron, at least max(4096, 16 + E0 x secondaries per eV)); a larger job
+ * returns MONSEL_E_TOOLARGE.  Calls on one GPU share the array (which keeps
+ * the previous call's contents; the tallies are cleared by each call) and
+ * would race, so the library runs them one at a time per GPU.  Its kernels
+ * share the context's stream with jobs submitted to the context.  Calls no
+ * hipMalloc, hipFree or other allocating routine. */
+int monsel_run_static(monsel_ctx ctx, monsel_scene_id scene, const monsel_job_desc* job,
+                      monsel_tallies* out, monsel_job_stats* stats);
 
 /* ---- probes: one device evaluation per input, for parity tests.  Inputs and
  * outputs are host arrays; energies in eV, lengths in nm. */
