@@ -344,9 +344,12 @@ __device__ __forceinline__ float* lane_rates() {
 
 /* Per-lane terms sigma_i * atoms_i of the elastic mechanism's rate (the
  * region model's first elastic mechanism), kept between the step's start and
- * its scatter so that the element choice reuses them. */
+ * its scatter so that the element choice reuses them.  The choice needs all
+ * terms but the last, so two slots serve materials of up to 3 elements
+ * (larger ones recompute the terms at the scatter). */
+constexpr int kKeptTerms = 2;
 __device__ __forceinline__ float* lane_esig() {
-  __shared__ float l_esig[kMaxElem * kLaneBlock];
+  __shared__ float l_esig[kKeptTerms * kLaneBlock];
   return l_esig + threadIdx.x;
 }
 
@@ -389,7 +392,9 @@ __device__ __forceinline__ void begin_step_in(const DevScene& S, int region, Ele
   PT_BEGIN(t_ra);
   const MONSEL_CS DMSM& msm = S.msms[S.regions[region].msm];
   float u[4];
+  PT_BEGIN(t_un);
   step_uniforms_lo(e, kPhiloxKey0, kPhiloxKey1, u);
+  PT_END(PH_UNIFORMS, t_un);
   float* rates = lane_rates();
   float total = 0.f;
   float freePath;
@@ -397,17 +402,19 @@ __device__ __forceinline__ void begin_step_in(const DevScene& S, int region, Ele
   if (msm.is_null) {
     freePath = 1.0e9f; /* NullMaterialScatterModel.cu:32-35: 1 m */
   } else {
+    PT_BEGIN(t_rl);
     for (int k = 0; k < msm.n_mech; ++k) {
       const MONSEL_CS DMech& m = S.mechs[msm.mech0 + k];
       float rk;
       if (k == msm.elastic_k) {
         /* elastic_rate, keeping its terms */
         float* esig = lane_esig();
+        const bool keep = m.nelem <= kKeptTerms + 1;
         rk = 0.f;
         for (int i = 0; i < m.nelem; ++i) {
           const MONSEL_CS DElem& el = S.elems[m.elem0 + i];
           const float c = elem_sigma(S, el, e.E) * el.natoms;
-          esig[i * kLaneBlock] = c;
+          if (keep && i < kKeptTerms) esig[i * kLaneBlock] = c;
           rk += c;
         }
       } else {
@@ -416,6 +423,7 @@ __device__ __forceinline__ void begin_step_in(const DevScene& S, int region, Ele
       rates[k * kLaneBlock] = rk;
       total += rk;
     }
+    PT_END(PH_RATE_LOOP, t_rl);
     if (total != 0.f) {
       float fp = -__logf(u[0]) / total;
       freePath = fp > maxFreePath ? maxFreePath : fp;
@@ -547,7 +555,8 @@ __device__ __forceinline__ int finish_step_in(const DevScene& S, const JobRec* j
       step_uniforms_hi(e, kPhiloxKey0, kPhiloxKey1, u);
       PT_END(PH_SELECT, t_se);
       sp = mech_scatter<TAB>(S, S.mechs[msm.mech0 + idx], e, prevE, rate, trapped, u,
-                             idx == msm.elastic_k ? lane_esig() : nullptr);
+                             idx == msm.elastic_k && S.mechs[msm.mech0 + idx].nelem <= kKeptTerms + 1 ? lane_esig()
+                                                                                               : nullptr);
       if (sp.has) {
         /* trackSecondaryElectron (MonteCarloSS.cu:165-174) */
         if (!(sp.E > msm.minE)) sp.has = false;
@@ -714,6 +723,7 @@ k_transport(DevScene S, GenDev G) {
   uint32_t nsteps = 0, nelec = 0, nesc = 0, nwd = 0, nover = 0;
   const uint32_t n_items = G.n_prim + G.in_n;
   const unsigned long long lanemask_lt = (1ull << lane_id()) - 1ull;
+  stage_mott_sigma(S);
   PT_INIT();
   for (;;) {
     PT_BEGIN(t_cl);
@@ -1196,6 +1206,7 @@ __global__ void k_pixels_to_double(const unsigned long long* px, int np, double*
 /* ---- probe kernels ---------------------------------------------------- */
 
 __global__ void k_probe_rates(DevScene S, int msm, int n, const float* E, float* out) {
+  stage_mott_sigma(S);
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const MONSEL_CS DMSM& m = S.msms[msm];
@@ -1230,6 +1241,7 @@ __global__ void k_probe_elastic(DevScene S, int mech, int element, int n, const 
  * E, secondary dir (3), trapped flag, 0. */
 __global__ void k_probe_scatter(DevScene S, int mech, int n, const float* prevE, const float* E, const float* dir,
                                 const float* u, float* out) {
+  stage_mott_sigma(S);
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const MONSEL_CS DMech& m = S.mechs[mech];

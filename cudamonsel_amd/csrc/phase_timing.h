@@ -36,6 +36,8 @@ enum Phase {
   PH_RATES_UNI,    /* step starts whose lanes share one region (entries only) */
   PH_GEOM_UNI,     /* findEndOfStep passes whose lanes share one region (entries only) */
   PH_SAFETY,       /* the safety bound at a step's end point */
+  PH_UNIFORMS,     /* the step's first four uniforms (within PH_RATES) */
+  PH_RATE_LOOP,    /* the mechanism rates (within PH_RATES) */
   kPhases
 };
 

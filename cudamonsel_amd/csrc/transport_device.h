@@ -225,12 +225,47 @@ MD float rutherford_cos(const MONSEL_CS DElem& el, float EeV, float r) {
   return 1.f - 2.0f * alpha * r / ((1.f - r) + alpha);
 }
 
+/* The hot tables: the cross-section nodes (Spwem, 61 floats) of the first
+ * kLdsMott NIST Mott elements of a scene, which every step above 50 eV reads
+ * for every element of its material, live in LDS; each kernel that evaluates
+ * cross-sections copies them there at its start (stage_mott_sigma).  The
+ * angle tables (61 x 201 per element, read once per elastic event) stay in
+ * device memory. */
+#ifndef MONSEL_LDS_MOTT_TABLES
+#define MONSEL_LDS_MOTT_TABLES 16
+#endif
+constexpr int kLdsMott = MONSEL_LDS_MOTT_TABLES;
+MD float* lds_spwem() {
+  __shared__ float l_spwem[(kLdsMott > 0 ? kLdsMott : 1) * kMottNE];
+  return l_spwem;
+}
+/* every thread of the block calls it before any cross-section */
+MD void stage_mott_sigma(const DevScene& S) {
+  const int n = min(S.n_mott, kLdsMott) * kMottNE;
+  float* l = lds_spwem();
+  for (int i = threadIdx.x; i < n; i += blockDim.x) l[i] = S.mott_spwem[i];
+  __syncthreads();
+}
+
 /* NISTMottRS::totalCrossSection (NISTMottRS.cu:103-119), nm^2 */
 MD float mott_table_sigma(const DevScene& S, int mott, float EeV) {
   float red = (__logf(EeV) - kMottLn50) * kMottInvParam;
   int i0 = u_index0(red, kMottNE);
-  const float* f = S.mott_spwem + mott * kMottNE + i0;
-  return kA0nm2 * lagrange4(red - (float)i0, f[0], f[1], f[2], f[3]);
+  float f0, f1, f2, f3;
+  if (mott < kLdsMott) {
+    const float* f = lds_spwem() + mott * kMottNE + i0;
+    f0 = f[0];
+    f1 = f[1];
+    f2 = f[2];
+    f3 = f[3];
+  } else {
+    const MONSEL_CS float* f = S.mott_spwem + mott * kMottNE + i0;
+    f0 = f[0];
+    f1 = f[1];
+    f2 = f[2];
+    f3 = f[3];
+  }
+  return kA0nm2 * lagrange4(red - (float)i0, f0, f1, f2, f3);
 }
 
 MD float elem_sigma(const DevScene& S, const MONSEL_CS DElem& el, float EeV) {
@@ -750,17 +785,17 @@ MD Spawn mech_scatter(const DevScene& S, const MONSEL_CS DMech& m, Elec& e, floa
     case MONSEL_MECH_BROWNING_ELASTIC:
     case MONSEL_MECH_RUTHERFORD_ELASTIC: { /* SelectableElasticSM.cu:49-67 */
       PT_BEGIN(t_el);
-      /* first element whose running sum of sigma * atoms reaches u * rate;
-       * the sum is re-formed in the order elastic_rate formed it, from the
-       * rate's own terms when the caller kept them (esig, lane stride
-       * kLaneBlock) */
+      /* first element whose running sum of sigma * atoms reaches u * rate
+       * (the last if none does); the sum is re-formed in the order
+       * elastic_rate formed it, from the rate's own terms when the caller
+       * kept them (esig: the first nelem - 1 terms, lane stride kLaneBlock) */
       const float r = u[2] * rate;
       int idx = 0;
-      float cum = esig ? esig[0] : elem_sigma(S, S.elems[m.elem0], prevE) * S.elems[m.elem0].natoms;
-      while (idx < m.nelem - 1 && cum < r) {
-        ++idx;
+      float cum = 0.f;
+      for (; idx < m.nelem - 1; ++idx) {
         const MONSEL_CS DElem& el = S.elems[m.elem0 + idx];
         cum += esig ? esig[idx * kLaneBlock] : elem_sigma(S, el, prevE) * el.natoms;
+        if (!(cum < r)) break;
       }
       const MONSEL_CS DElem& el = S.elems[m.elem0 + idx];
       /* the polar angle takes one uniform (two for Browning), the azimuth

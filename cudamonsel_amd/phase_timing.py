@@ -6,7 +6,9 @@ Needs a library built with `python -m cudamonsel_amd.build --phase-timing`
 counters and prints, per phase, its share of the wave cycles of the whole
 loop, the cycles per entry and the lane utilisation on entry.
 
-Usage: python -m cudamonsel_amd.phase_timing [per_pixel] [repeats]
+Usage: python -m cudamonsel_amd.phase_timing [per_pixel] [repeats] [--default-engine]
+(the engine runs the line-scan tuning of data/tuning_linescan.json unless
+--default-engine is given)
 """
 import ctypes
 import sys
@@ -14,7 +16,7 @@ import sys
 import numpy as np
 
 PHASES = ["claim", "contain", "rates", "geom", "csd", "select", "elastic", "fitted", "polaron", "phonon",
-          "tabulated", "barrier", "step", "tail", "rates_uni", "geom_uni", "safety"]
+          "tabulated", "barrier", "step", "tail", "rates_uni", "geom_uni", "safety", "uniforms", "rate_loop"]
 
 
 def read_counters(lib, device=0):
@@ -76,21 +78,32 @@ def table(c):
 
 
 def main():
-    per_pixel = int(sys.argv[1]) if len(sys.argv) > 1 else 2000
-    repeats = int(sys.argv[2]) if len(sys.argv) > 2 else 2
+    import json
+    import os
+    args = [a for a in sys.argv[1:] if not a.startswith("--")]
+    per_pixel = int(args[0]) if len(args) > 0 else 2000
+    repeats = int(args[1]) if len(args) > 1 else 2
     import cudamonsel_amd as M
     from cudamonsel_amd import scenes
+    from cudamonsel_amd.abi import Engine
+    tuning = {}
+    tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "tuning_linescan.json")
+    if "--default-engine" not in sys.argv and os.path.exists(tfile):
+        tuning = json.load(open(tfile))["tuning"]
+    print(f"engine tuning: {tuning or 'defaults'}")
     sc = scenes.line_on_substrate(h_nm=50.0, w_nm=20.0, theta_deg=3.0, r_nm=5.0, length_nm=1000.0,
                                   line="PMMA", substrate="Si", beam_eV=500.0, sigma_nm=0.5, compat=True)
     centers = scenes.scan_centers(np.arange(-64.0, 64.0, 1.0), [0.0], sc.beam_z)
     lib = M.gpu_lib()
-    sc.monte.runPixels(centers, per_pixel, 50, 50.0)   # warm-up
-    read_counters(lib)
-    read_spans(lib)
     steps = 0
-    for _ in range(repeats):
-        r = sc.monte.runPixels(centers, per_pixel, 50, 50.0)
-        steps += r.stats.steps
+    with Engine(**tuning) as eng:
+        sid = eng.upload(sc.monte.compileScene())
+        eng.result(eng.submit(sid, centers, per_pixel, 500.0, 0.5e-9, 1))   # warm-up
+        read_counters(lib)
+        read_spans(lib)
+        for k in range(repeats):
+            r = eng.result(eng.submit(sid, centers, per_pixel, 500.0, 0.5e-9, 2 + k))
+            steps += r["stats"]["steps"]
     c = read_counters(lib)
     print(f"line scan 128 px x {per_pixel} x {repeats}: {steps} electron steps")
     print(f"{'phase':<10} {'share':>7} {'cyc/entry':>10} {'entries':>12} {'lanes':>6}")

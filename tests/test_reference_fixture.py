@@ -4,12 +4,13 @@ tests/golden/reference_run_log.json by tests/golden/make_reference_run_log.py).
 
 The log is a 0.5 keV PMMA line scan with 100 primaries per pixel from an
 unrecorded revision of the reference; far from the line (y > 700 nm) the
-surface is flat PMMA.  Its BSE yield is compared within 4 standard errors,
-its SE yield -- counted as E <= 40 eV through the histogram defect -- within
-2 % (the oracle sits 1.3 % above the log whichever random-number generator
-model it uses: Philox 2.2135, 15-bit uniforms 2.2200, the MSVC rand() LCG
-2.2176, log 2.1888).  Smoke-level, as the log's revision is unknown; the
-3-sigma gates are GPU against the oracle."""
+surface is flat PMMA.  Its BSE yield is compared within 4 standard errors.
+Its SE yield -- counted as E < 40 eV through the histogram defect -- is
+1.0109 +- 0.0007 times the log's in the oracle, uniformly over the log's
+whole line profile (measured on the log's own geometry, 80 columns x 29,600
+primaries; no reference-mode toggle, random-number model or element order
+accounts for it), so the SE yield is compared with 1.0109 x the log's within
+4 standard errors of the two runs."""
 import json
 import os
 
@@ -25,12 +26,13 @@ def test_flat_pmma_yields_match_reference_log(oracle, M):
     from cudamonsel_amd import scenes
     sc = scenes.flat("PMMA", beam_eV=500.0)
     osc = oracle.OracleScene(sc.monte.compileScene())
-    npx, per = 48, 500
+    npx, per = 64, 1500
     centers = np.array([[0.0, 0.0, sc.beam_z]] * npx)
     r = osc.run(centers, per, M.ToSI.eV(500.0), 0.5e-9, seed=99, nbins=50, threads=8)
     n = npx * per
     se = r["se"].sum() / n
     bse = (r["back"].sum() - r["se"].sum()) / n
     bse_err = np.sqrt((r["bseSq"].sum() / n - bse ** 2) / n)
+    se_err = np.sqrt((r["seSq"].sum() / n - se ** 2) / n)
     assert abs(bse - g["bse_mean"]) < 4 * np.hypot(bse_err, g["bse_sem"])
-    assert abs(se / g["se_mean"] - 1.0) < 0.02
+    assert abs(se - 1.0109 * g["se_mean"]) < 4 * np.hypot(se_err, 1.0109 * g["se_sem"]), se / g["se_mean"]
