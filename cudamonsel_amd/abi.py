@@ -19,7 +19,22 @@ class JobDesc(ctypes.Structure):
         ("pixel_center", c_double_p), ("seed", ctypes.c_uint64),
         ("n_energy_bins", ctypes.c_int32), ("max_steps", ctypes.c_int32),
         ("se_cutoff_ev", ctypes.c_double), ("primary_offset", ctypes.c_uint64),
+        ("log_detected", ctypes.c_int64),
     ]
+
+
+class Detected(ctypes.Structure):
+    """monsel_detected"""
+    _fields_ = [
+        ("electron", ctypes.c_uint64), ("primary", ctypes.c_uint32), ("step", ctypes.c_uint32),
+        ("energy_ev", ctypes.c_double), ("position", ctypes.c_double * 3), ("theta", ctypes.c_double),
+        ("phi", ctypes.c_double),
+    ]
+
+
+DETECTED_DTYPE = np.dtype([("electron", np.uint64), ("primary", np.uint32), ("step", np.uint32),
+                           ("energy_ev", np.float64), ("position", np.float64, (3,)), ("theta", np.float64),
+                           ("phi", np.float64)])
 
 
 class Tallies(ctypes.Structure):
@@ -59,7 +74,7 @@ SYMBOLS = [
     "monsel_job_release", "monsel_run", "monsel_probe_rates", "monsel_probe_elastic_angle", "monsel_probe_csd",
     "monsel_probe_find_end_of_step", "monsel_probe_locate", "monsel_probe_nu_table", "monsel_probe_scatter",
     "monsel_probe_barrier", "monsel_probe_safety", "monsel_job_cancel", "monsel_probe_philox",
-    "monsel_probe_hist_bin", "monsel_probe_primaries", "monsel_run_static",
+    "monsel_probe_hist_bin", "monsel_probe_primaries", "monsel_run_static", "monsel_job_read_detected",
 ]
 
 
@@ -233,14 +248,15 @@ class Engine:
         return sid.value
 
     def submit(self, scene_id, centers_m, per_pixel, beam_eV, sigma_m, seed, nbins=50, se_cutoff_eV=50.0,
-               max_steps=0, primary_offset=0):
+               max_steps=0, primary_offset=0, log_detected=0):
         c = np.ascontiguousarray(centers_m, dtype=np.float64).reshape(-1, 3)
         jd = JobDesc(beam_eV * 1.60217653e-19, sigma_m, c.shape[0], int(per_pixel), c.ctypes.data_as(c_double_p),
-                     int(seed), int(nbins), int(max_steps), float(se_cutoff_eV), int(primary_offset))
+                     int(seed), int(nbins), int(max_steps), float(se_cutoff_eV), int(primary_offset),
+                     int(log_detected))
         jid = ctypes.c_uint64()
         _check(self.L.monsel_job_submit(self.ctx, ctypes.c_uint64(scene_id), ctypes.byref(jd), ctypes.byref(jid)),
                "monsel_job_submit")
-        self._keep[jid.value] = (c.shape[0], int(nbins))
+        self._keep[jid.value] = (c.shape[0], int(nbins), int(log_detected))
         return jid.value
 
     def counters(self):
@@ -285,7 +301,7 @@ class Engine:
         c = np.ascontiguousarray(centers_m, dtype=np.float64).reshape(-1, 3)
         n = c.shape[0]
         jd = JobDesc(beam_eV * 1.60217653e-19, sigma_m, n, int(per_pixel), c.ctypes.data_as(c_double_p),
-                     int(seed), int(nbins), int(max_steps), float(se_cutoff_eV), 0)
+                     int(seed), int(nbins), int(max_steps), float(se_cutoff_eV), 0, 0)
         r, t = self._tally_buffers(n, int(nbins))
         st = JobStats()
         fn = self.L.monsel_run_static if static else self.L.monsel_run
@@ -313,11 +329,18 @@ class Engine:
         """Waits for the job, returns its tallies and statistics, releases it."""
         us = -1 if timeout_s is None else int(timeout_s * 1e6)
         _check(self.L.monsel_job_wait(self.ctx, ctypes.c_uint64(job_id), ctypes.c_int64(us)), "monsel_job_wait")
-        n, nb = self._keep.pop(job_id)
+        n, nb, nlog = self._keep.pop(job_id)
         r, t = self._tally_buffers(n, nb)
         _check(self.L.monsel_job_read(self.ctx, ctypes.c_uint64(job_id), ctypes.byref(t)), "monsel_job_read")
         st = JobStats()
         _check(self.L.monsel_job_stats_get(self.ctx, ctypes.c_uint64(job_id), ctypes.byref(st)), "monsel_job_stats_get")
         r["stats"] = {f: getattr(st, f) for f, _ in JobStats._fields_}
+        if nlog > 0:
+            buf = np.zeros(nlog, DETECTED_DTYPE)
+            nd = ctypes.c_int64(0)
+            _check(self.L.monsel_job_read_detected(self.ctx, ctypes.c_uint64(job_id), buf.ctypes.data_as(ctypes.c_void_p),
+                                                   ctypes.c_int64(nlog), ctypes.byref(nd)), "monsel_job_read_detected")
+            r["detected"] = buf[:min(nlog, nd.value)]
+            r["n_detected"] = nd.value
         self.L.monsel_job_release(self.ctx, ctypes.c_uint64(job_id))
         return r

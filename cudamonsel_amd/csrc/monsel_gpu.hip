@@ -73,7 +73,7 @@ struct MonselError {
 constexpr int kQF = 7; /* float fields per queued electron: p, d, E */
 constexpr int kQU = 7; /* u32 fields: region, primary, lineage lo/hi, job slot, steps, children */
 constexpr int kEF = 7; /* float fields per escape record: tally origin, d, E */
-constexpr int kEU = 2; /* u32 fields: primary, job slot << 1 | through-the-chamber-wall flag */
+constexpr int kEU = 5; /* u32 fields: primary, job slot << 1 | through-the-chamber-wall flag, lineage lo/hi, steps */
 constexpr int kStatSteps = 0, kStatElectrons = 1, kStatEscapes = 2, kStatWatchdog = 3, kStatOverflow = 4;
 /* a job's counters are kept in stripes of 8 u64 (waves add to the stripe of
  * their index), so that waves of many blocks do not contend for one address;
@@ -114,6 +114,9 @@ struct JobRec {
   unsigned long long* stats; /* [8] */
   float* wd_log;           /* optional: state of the first kWatchdogLog watchdog drops */
   unsigned int* wd_count;
+  monsel_detected* det_log; /* optional: detected-electron records (det_max of them) */
+  unsigned long long* det_count;
+  unsigned long long det_max;
 };
 constexpr int kWatchdogLog = 16, kWatchdogFields = 10;
 
@@ -210,7 +213,7 @@ __host__ __device__ __forceinline__ int hist_len(int nbins) { return 2 * (nbins 
  * Histogram counts go to `lh` (a block-private LDS copy) when given. */
 __device__ __forceinline__ void tally_escape(const DevScene& S, const JobRec& J, uint32_t primary, float E, float px,
                                              float py, float pz, float dx, float dy, float dz, bool fromRoot,
-                                             unsigned int* lh) {
+                                             unsigned int* lh, uint64_t lineage, uint32_t steps) {
   double x = px, y = py, z = pz;
   if (fromRoot) {
     double cx = x - J.chamber_c[0], cy = y - J.chamber_c[1], cz = z - J.chamber_c[2];
@@ -248,6 +251,23 @@ __device__ __forceinline__ void tally_escape(const DevScene& S, const JobRec& J,
     atomicAdd(&J.hist[o_elev], 1ull);
     atomicAdd(&J.hist[o_azim], 1ull);
     atomicAdd(&J.hist[o_e], 1ull);
+  }
+  if (J.det_log) {
+    /* BackscatterStats.cu:166-168: the detected electron's Datum */
+    const unsigned long long k = atomicAdd(J.det_count, 1ull);
+    if (k < J.det_max) {
+      monsel_detected* d = J.det_log + k;
+      const double dn = sqrt((double)dx * dx + (double)dy * dy + (double)dz * dz);
+      d->electron = lineage;
+      d->primary = primary;
+      d->step = steps;
+      d->energy_ev = (double)E;
+      d->position[0] = x * 1e-9;
+      d->position[1] = y * 1e-9;
+      d->position[2] = z * 1e-9;
+      d->theta = acos(fmin(1.0, fmax(-1.0, (double)dz / dn)));
+      d->phi = atan2((double)dy, (double)dx);
+    }
   }
   const uint32_t rel = primary - J.prim_base;
   if (fwd) {
@@ -872,6 +892,9 @@ k_transport(DevScene S, GenDev G) {
           G.esc_f[6 * cap + slot] = e.E;
           G.esc_u[0 * cap + slot] = e.primary;
           G.esc_u[1 * cap + slot] = (e.job << 1) | (uint32_t)e.region;
+          G.esc_u[2 * cap + slot] = e.lin_lo;
+          G.esc_u[3 * cap + slot] = e.lin_hi;
+          G.esc_u[4 * cap + slot] = e.steps;
         } else {
           nover++;
         }
@@ -1019,7 +1042,8 @@ __global__ void __launch_bounds__(256) k_tally_escapes(DevScene S, const JobRec*
     for (int k = 0; k < lj.n; ++k)
       if (lj.slot[k] == job) own = lh + lj.off[k];
     tally_escape(S, jobs[job], eu[i], ef[6 * cap + i], ef[i], ef[cap + i], ef[2 * cap + i], ef[3 * cap + i],
-                 ef[4 * cap + i], ef[5 * cap + i], (w & 1u) != 0u, own);
+                 ef[4 * cap + i], ef[5 * cap + i], (w & 1u) != 0u, own,
+                 ((uint64_t)eu[3 * cap + i] << 32) | eu[2 * cap + i], eu[4 * cap + i]);
   }
   __syncthreads();
   for (int k = 0; k < lj.n; ++k) {
@@ -2024,7 +2048,9 @@ struct Job {
   uint64_t reserved = 0;      /* queue entries reserved for its electrons in flight */
   uint32_t outstanding = 0;   /* its electrons in the current queue */
   int generations = 0;
-  DBuf prim_counts, prim_fwd, prim_energy, px, hist, stats_dev, centers_dev, wd;
+  DBuf prim_counts, prim_fwd, prim_energy, px, hist, stats_dev, centers_dev, wd, det;
+  std::vector<monsel_detected> detected; /* the kept records, after completion */
+  uint64_t n_detected = 0;
 };
 
 }  // namespace
@@ -2192,6 +2218,14 @@ void setup_job(monsel_ctx_s* C, Job& job) {
     R.wd_log = job.wd.as<float>();
     R.wd_count = (unsigned int*)(job.wd.as<float>() + kWatchdogLog * kWatchdogFields);
   }
+  if (jd.log_detected > 0) {
+    const size_t bytes = 16 + (size_t)jd.log_detected * sizeof(monsel_detected);
+    job.det.ensure(bytes);
+    HIP_TRY(hipMemsetAsync(job.det.p, 0, 16, C->stream));
+    R.det_count = job.det.as<unsigned long long>();
+    R.det_log = (monsel_detected*)(job.det.as<char>() + 16);
+    R.det_max = (unsigned long long)jd.log_detected;
+  }
   HIP_TRY(hipMemcpyAsync(C->jobtab.as<JobRec>() + job.slot, &R, sizeof(JobRec), hipMemcpyHostToDevice, C->stream));
   HIP_TRY(hipStreamSynchronize(C->stream));
 }
@@ -2231,6 +2265,16 @@ void complete_job(monsel_ctx_s* C, Job& job) {
   job.hfwd.assign(h.begin() + nb2, h.begin() + 2 * nb2);
   job.helev.assign(h.begin() + 2 * nb2, h.begin() + 2 * nb2 + 182);
   job.hazim.assign(h.begin() + 2 * nb2 + 182, h.begin() + 2 * nb2 + 182 + 362);
+  if (job.det.p) {
+    unsigned long long nd = 0;
+    HIP_TRY(hipMemcpy(&nd, job.det.p, 8, hipMemcpyDeviceToHost));
+    job.n_detected = nd;
+    const size_t kept = (size_t)std::min<unsigned long long>(nd, (unsigned long long)jd.log_detected);
+    job.detected.resize(kept);
+    if (kept)
+      HIP_TRY(hipMemcpy(job.detected.data(), job.det.as<char>() + 16, kept * sizeof(monsel_detected),
+                        hipMemcpyDeviceToHost));
+  }
   job.stats.primaries = job.total;
   job.stats.steps = st[kStatSteps];
   {
@@ -2269,6 +2313,7 @@ void release_job(monsel_ctx_s* C, Job& job) {
   job.stats_dev.reset();
   job.centers_dev.reset();
   job.wd.reset();
+  job.det.reset();
   job.sd.reset();
 }
 
@@ -2716,7 +2761,8 @@ int monsel_job_footprint(monsel_ctx ctx, const monsel_job_desc* job, size_t* hbm
   const uint64_t resv = std::min<uint64_t>(total * K, ctx->qcap);
   const int nbins = job->n_energy_bins > 0 ? job->n_energy_bins : 1;
   *hbm_bytes = (size_t)(total * 16 + (uint64_t)job->n_pixels * (6 * 8 + 24) + (uint64_t)hist_len(nbins) * 8 + 64 +
-                        resv * (2 * kQueueEntryBytes + kEscEntryBytes));
+                        resv * (2 * kQueueEntryBytes + kEscEntryBytes) +
+                        (job->log_detected > 0 ? 16 + (uint64_t)job->log_detected * sizeof(monsel_detected) : 0));
   return MONSEL_OK;
 }
 
@@ -2729,6 +2775,8 @@ static int validate_job(const monsel_job_desc* job) {
   uint64_t total = (uint64_t)job->n_pixels * (uint64_t)job->primaries_per_pixel;
   if (total + job->primary_offset > 0xFFFFFFFFull) return set_err(MONSEL_E_INVALID, "more than 2^32 primaries in one job");
   if (job->n_energy_bins > (1 << 20)) return set_err(MONSEL_E_INVALID, "too many energy bins");
+  if (job->log_detected < 0 || job->log_detected > ((int64_t)1 << 32))
+    return set_err(MONSEL_E_INVALID, "log_detected must be in [0, 2^32]");
   return MONSEL_OK;
 }
 
@@ -2813,6 +2861,24 @@ int monsel_job_stats_get(monsel_ctx ctx, monsel_job_id job, monsel_job_stats* ou
   return MONSEL_OK;
 }
 
+int monsel_job_read_detected(monsel_ctx ctx, monsel_job_id job, monsel_detected* out, int64_t max,
+                             int64_t* n_detected) {
+  if (!ctx || !n_detected || (max > 0 && !out)) return set_err(MONSEL_E_INVALID, "NULL argument");
+  std::shared_ptr<Job> j;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->jobs.find(job);
+    if (it == ctx->jobs.end()) return set_err(MONSEL_E_NOTFOUND, "unknown job");
+    j = it->second;
+    if (!j->done) return set_err(MONSEL_E_NOTREADY, "job still running");
+    if (j->status != MONSEL_OK) return set_err(j->status, j->error);
+  }
+  *n_detected = (int64_t)j->n_detected;
+  const size_t k = std::min<size_t>(j->detected.size(), max > 0 ? (size_t)max : 0);
+  if (k) std::copy(j->detected.begin(), j->detected.begin() + k, out);
+  return MONSEL_OK;
+}
+
 int monsel_job_release(monsel_ctx ctx, monsel_job_id job) {
   if (!ctx) return set_err(MONSEL_E_INVALID, "NULL context");
   std::lock_guard<std::mutex> lk(ctx->mu);
@@ -2868,6 +2934,8 @@ int monsel_run_static(monsel_ctx ctx, monsel_scene_id scene, const monsel_job_de
   if (!ctx) return set_err(MONSEL_E_INVALID, "NULL context");
   const int v = validate_job(job);
   if (v) return v;
+  if (job->log_detected > 0)
+    return set_err(MONSEL_E_UNSUPPORTED, "monsel_run_static keeps no detected-electron log");
   if (ctx->device >= kMaxStaticDevices)
     return set_err(MONSEL_E_UNSUPPORTED, "monsel_run_static: device ordinal above " +
                                              std::to_string(kMaxStaticDevices - 1));

@@ -9,10 +9,20 @@ Inputs (read-only, from the reference checkout):
   gov/nist/microanalysis/EPQLibrary/AtomicWeights.csv  one weight per line,
       '//' comment header (read by Element.cu:593-620 as float)
 
+  gov/nist/microanalysis/EPQLibrary/FFastEdgeDB.csv  edge energies (eV) per
+      element Z = 1..92, one row each, shells K, L1-3, M1-5, N1-7, O1-5,
+      P1-3 (read by ChantlerEdgeEnergy::loadFFastEdgeDB, EdgeEnergy.cu:153-177,
+      which keeps each row's non-zero values)
+  gov/nist/microanalysis/EPQLibrary/ElectronConfig.csv  ground-state shell
+      occupancies (ENDLIB97) per element, "Z, n_K, n_L1, ..." with '#'
+      comments (AtomicShell.cu:386-412)
+
 Outputs (written next to this script):
   nistmott.bin        little-endian: b"NMOT", u32 version=1, u32 count,
                       then per element: i32 Z, f64 spwem[61], f64 x1[61*201]
   atomic_weights.txt  one weight per line for Z = 1.., as text
+  edge_energies.txt   line Z-1: the element's non-zero edge energies (eV)
+  electron_config.txt line Z-1: the element's shell occupancies
 
 Usage: python make_data.py [reference_root] [Z ...]
 """
@@ -64,7 +74,24 @@ def main(argv):
             weights.append(s.split(",")[0])
     with open(os.path.join(here, "atomic_weights.txt"), "w") as f:
         f.write("\n".join(weights) + "\n")
-    print(f"wrote {len(zs)} Mott tables, {len(weights)} atomic weights")
+    edges = []
+    with open(os.path.join(root, EPQ, "FFastEdgeDB.csv")) as f:
+        for line in f:
+            vals = [float(v) for v in line.strip().split(",") if v.strip()]
+            edges.append(" ".join(repr(v) for v in vals if v != 0.0))
+    with open(os.path.join(here, "edge_energies.txt"), "w") as f:
+        f.write("\n".join(edges) + "\n")
+    occ = []
+    with open(os.path.join(root, EPQ, "ElectronConfig.csv")) as f:
+        for line in f:
+            if not line.strip() or line.lstrip().startswith("#"):
+                continue
+            vals = [int(v) for v in line.strip().split(",") if v.strip()]
+            occ.append(" ".join(str(v) for v in vals[1:]))
+    with open(os.path.join(here, "electron_config.txt"), "w") as f:
+        f.write("\n".join(occ) + "\n")
+    print(f"wrote {len(zs)} Mott tables, {len(weights)} atomic weights, {len(edges)} edge rows, "
+          f"{len(occ)} occupancy rows")
 
 
 if __name__ == "__main__":

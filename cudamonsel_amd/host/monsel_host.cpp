@@ -214,6 +214,70 @@ void SEmaterial::setWorkfunction(double w) {
   if (energyCBbottom_ > -w) energyCBbottom_ = -w;
 }
 
+/* Rows of whitespace-separated numbers, one per element (line Z-1). */
+static std::vector<std::vector<double>> readRows(const std::string& file) {
+  std::vector<std::vector<double>> rows;
+  std::ifstream f(getDataDirectory() + "/" + file);
+  if (!f) throw Error(MONSEL_E_INVALID, "cannot read " + getDataDirectory() + "/" + file);
+  std::string line;
+  while (std::getline(f, line)) {
+    std::istringstream is(line);
+    std::vector<double> r;
+    double v;
+    while (is >> v) r.push_back(v);
+    rows.push_back(r);
+  }
+  return rows;
+}
+
+/* ChantlerEdgeEnergy (EdgeEnergy.cu:153-205): per element the non-zero edge
+ * energies (eV) of shells K .. O5 then P1 .. P3. */
+static const std::vector<std::vector<double>>& chantlerEdges() {
+  static std::vector<std::vector<double>> rows;
+  static std::once_flag once;
+  std::call_once(once, [] { rows = readRows("edge_energies.txt"); });
+  return rows;
+}
+
+/* AtomicShell::getGroundStateOccupancy (AtomicShell.cu:386-425). */
+static const std::vector<std::vector<double>>& shellOccupancy() {
+  static std::vector<std::vector<double>> rows;
+  static std::once_flag once;
+  std::call_once(once, [] { rows = readRows("electron_config.txt"); });
+  return rows;
+}
+
+/* EdgeEnergy::SuperSet = Chantler 2005 for Z <= 92 (its compute returns 0
+ * rather than throwing, so the NIST and DTSA fallbacks are never reached),
+ * in J.  The shell numbering is AtomicShell's: K = 0, ..., O5 = 20,
+ * O6 .. O9 = 21 .. 24 (no Chantler column), P1 = 25. */
+static double chantlerEdgeEnergy(int Z, int shell) {
+  const auto& E = chantlerEdges();
+  int i = shell <= 20 ? shell : (shell >= 25 ? shell - 4 : -1);
+  if (Z < 1 || Z > (int)E.size() || i < 0 || i >= (int)E[Z - 1].size()) return 0.0;
+  return ToSI::eV(E[Z - 1][i]);
+}
+
+void SEmaterial::setEstimatedCoreEnergy() { setEstimatedCoreEnergy(ToSI::eV(20.)); }
+
+void SEmaterial::setEstimatedCoreEnergy(double cutoff) {
+  core_.clear();
+  const auto& occ = shellOccupancy();
+  for (int Z : elements_) {
+    if (Z < 1 || Z > (int)occ.size()) throw Error(MONSEL_E_INVALID, "no shell occupancy for Z=" + std::to_string(Z));
+    const auto& o = occ[Z - 1];
+    /* SEmaterial.cu:357-364: occupied shells, innermost first, until one's
+     * edge is at or below the cutoff.  The reference would go on past the
+     * last occupied shell (an unoccupied shell gives NaN, which never meets
+     * the cutoff); this stops there. */
+    for (int shell = 0; shell < (int)o.size() && o[shell] > 0; ++shell) {
+      const double e = chantlerEdgeEnergy(Z, shell);
+      if (e <= cutoff) break;
+      addCoreEnergy(e);
+    }
+  }
+}
+
 void SEmaterial::setCoreEnergy(const double coreEnergy[], int len) {
   core_.clear();
   for (int i = 0; i < len; ++i) {
@@ -579,6 +643,15 @@ void BackscatterStats::jobCompleted(const MonteCarloSS& mc, int n, const monsel_
   eventCount_ = n;
 }
 
+void BackscatterStats::detectedLog(const std::vector<monsel_detected>& log, int64_t nDetected) {
+  log_.clear();
+  log_.reserve(log.size());
+  for (const monsel_detected& d : log)
+    log_.push_back(Datum{d.electron, d.primary, (long)d.step, d.energy_ev,
+                         {d.position[0], d.position[1], d.position[2]}, d.theta, d.phi});
+  logTruncated_ = nDetected > (int64_t)log.size();
+}
+
 /* ------------------------------------------------------- serialisation -- */
 
 static void fnv(uint64_t& h, const void* p, size_t n) {
@@ -885,12 +958,36 @@ PixelResults MonteCarloSS::runPixels(const std::vector<std::array<double, 3>>& c
   t.fwd_energy_hist = pr.fwdEnergyHist.data();
   t.elevation_hist = pr.elevationHist.data();
   t.azimuth_hist = pr.azimuthHist.data();
-  check(monsel_run(dev.ctx(), sid, &job, &t, &pr.stats), "monsel_run");
+  int64_t logCap = 0;
+  for (ActionListener* l : listeners_) logCap = std::max(logCap, l->detectedCapacity(nPerPixel * pr.nPixels));
+  std::vector<monsel_detected> detected;
+  int64_t nDetected = 0;
+  if (logCap == 0) {
+    check(monsel_run(dev.ctx(), sid, &job, &t, &pr.stats), "monsel_run");
+  } else {
+    job.log_detected = logCap;
+    monsel_job_id jid = 0;
+    check(monsel_job_submit(dev.ctx(), sid, &job, &jid), "monsel_job_submit");
+    int st = monsel_job_wait(dev.ctx(), jid, -1);
+    if (st == MONSEL_OK) st = monsel_job_read(dev.ctx(), jid, &t);
+    if (st == MONSEL_OK) st = monsel_job_stats_get(dev.ctx(), jid, &pr.stats);
+    if (st == MONSEL_OK) {
+      detected.resize((size_t)logCap);
+      st = monsel_job_read_detected(dev.ctx(), jid, detected.data(), logCap, &nDetected);
+      detected.resize((size_t)std::min<int64_t>(nDetected, logCap));
+    }
+    std::string err = st == MONSEL_OK ? std::string() : monsel_last_error();
+    monsel_job_release(dev.ctx(), jid);
+    if (st != MONSEL_OK) throw Error(st, "monsel_job: " + err);
+  }
   stats_ = pr.stats;
   /* advance the seed so successive runs draw fresh streams, like the
    * continuing rand() sequence of the reference */
   seed_ = seed_ * 6364136223846793005ull + 1442695040888963407ull;
-  for (ActionListener* l : listeners_) l->jobCompleted(*this, nPerPixel * pr.nPixels, t);
+  for (ActionListener* l : listeners_) {
+    l->jobCompleted(*this, nPerPixel * pr.nPixels, t);
+    if (logCap > 0) l->detectedLog(detected, nDetected);
+  }
   return pr;
 }
 

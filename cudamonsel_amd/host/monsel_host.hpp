@@ -117,6 +117,13 @@ class SEmaterial : public Material {
   void setBandgap(double g) { bandgap_ = g; }
   void setCoreEnergy(const double coreEnergy[], int len);
   void addCoreEnergy(double e) { core_.insert(e); }
+  /* SEmaterial::setEstimatedCoreEnergy (SEmaterial.cu:347-366): the core
+   * energies from the constituent elements' edge energies (NIST-Chantler
+   * 2005, FFastEdgeDB) of their occupied shells, innermost first, down to
+   * the first occupied shell whose edge is at or below the cutoff (J;
+   * default 20 eV). */
+  void setEstimatedCoreEnergy();
+  void setEstimatedCoreEnergy(double cutoff);
   double getWorkfunction() const { return workfunction_; }
   double getEnergyCBbottom() const { return energyCBbottom_; }
   double getBandgap() const { return bandgap_; }
@@ -464,12 +471,47 @@ class ActionListener {
   /* Called once per run with the job's device tallies. */
   virtual void jobCompleted(const MonteCarloSS& mc, int nTrajectories, const monsel_tallies& t) = 0;
   virtual int energyBins() const { return 0; }
+  /* Detected-electron records wanted from a run of nTrajectories (0: none),
+   * and the records of the run (kept ones, and how many were detected). */
+  virtual int64_t detectedCapacity(int nTrajectories) const { (void)nTrajectories; return 0; }
+  virtual void detectedLog(const std::vector<monsel_detected>& log, int64_t nDetected) {
+    (void)log;
+    (void)nDetected;
+  }
 };
 
-/* BackscatterStats.cu:107-118, 252-270 */
+/* BackscatterStats::Datum (BackscatterStats.cu:40-48): one detected electron.
+ * The electron's identity is its lineage within the cascade of `primary`. */
+struct Datum {
+  uint64_t electronID;
+  uint32_t primary;
+  long trajStep;
+  double energyeV;
+  std::array<double, 3> position; /* m */
+  double theta, phi;
+  long getElectronID() const { return (long)electronID; }
+  long getTrajStep() const { return trajStep; }
+  double getEnergyeV() const { return energyeV; }
+  std::array<double, 3> getPosition() const { return position; }
+  double getTheta() const { return theta; }
+  double getPhi() const { return phi; }
+};
+
+/* BackscatterStats.cu:107-118, 166-168, 252-270 */
 class BackscatterStats : public ActionListener {
  public:
   BackscatterStats(const MonteCarloSS& mcss, int nEnergyBins);
+  /* BackscatterStats::setLogDetected: keep a Datum per detected electron
+   * (at most 64 per trajectory and 2^26 per run are kept; logTruncated()
+   * tells when a run detected more). */
+  void setLogDetected(bool logDetected) { logDetected_ = logDetected; }
+  bool getLogDetected() const { return logDetected_; }
+  const std::vector<Datum>& getLog() const { return log_; }
+  bool logTruncated() const { return logTruncated_; }
+  int64_t detectedCapacity(int nTrajectories) const override {
+    return logDetected_ ? std::min<int64_t>((int64_t)64 * nTrajectories, (int64_t)1 << 26) : 0;
+  }
+  void detectedLog(const std::vector<monsel_detected>& log, int64_t nDetected) override;
   double backscatterFraction() const;
   double forwardscatterFraction() const;
   double scatterFraction() const { return backscatterFraction() + forwardscatterFraction(); }
@@ -486,6 +528,9 @@ class BackscatterStats : public ActionListener {
   double beamEnergyEV_;
   long eventCount_ = 0;
   Histogram elev_, azim_, fwd_, back_;
+  bool logDetected_ = false;
+  bool logTruncated_ = false;
+  std::vector<Datum> log_;
 };
 
 /* Per-pixel results of a scan (the quantities LinesOnLayers.cu:366-376 derives). */

@@ -86,6 +86,11 @@ PYBIND11_MODULE(_monsel, m) {
       .def("setBandgap", &SEmaterial::setBandgap)
       .def("setCoreEnergy",
            [](SEmaterial& s, const std::vector<double>& c) { s.setCoreEnergy(c.data(), (int)c.size()); })
+      .def("setEstimatedCoreEnergy", py::overload_cast<>(&SEmaterial::setEstimatedCoreEnergy))
+      .def("setEstimatedCoreEnergy", py::overload_cast<double>(&SEmaterial::setEstimatedCoreEnergy))
+      .def("getCoreEnergyArray", [](const SEmaterial& m) {
+        return std::vector<double>(m.getCoreEnergyArray().begin(), m.getCoreEnergyArray().end());
+      })
       .def("getWorkfunction", &SEmaterial::getWorkfunction)
       .def("getEnergyCBbottom", &SEmaterial::getEnergyCBbottom)
       .def("getBandgap", &SEmaterial::getBandgap)
@@ -211,7 +216,23 @@ PYBIND11_MODULE(_monsel, m) {
            py::return_value_policy::reference_internal)
       .def("elevationHistogram", &BackscatterStats::elevationHistogram, py::return_value_policy::reference_internal)
       .def("azimuthalHistogram", &BackscatterStats::azimuthalHistogram, py::return_value_policy::reference_internal)
-      .def("getEnergyBinCount", &BackscatterStats::getEnergyBinCount);
+      .def("getEnergyBinCount", &BackscatterStats::getEnergyBinCount)
+      .def("setLogDetected", &BackscatterStats::setLogDetected)
+      .def("getLogDetected", &BackscatterStats::getLogDetected)
+      .def("logTruncated", &BackscatterStats::logTruncated)
+      .def("getLog", [](const BackscatterStats& b) {
+        /* columns: electron id, primary, step, energy (eV), x, y, z (m), theta, phi */
+        const auto& L = b.getLog();
+        py::array_t<double> a({(py::ssize_t)L.size(), (py::ssize_t)9});
+        auto r = a.mutable_unchecked<2>();
+        for (size_t i = 0; i < L.size(); ++i) {
+          const Datum& d = L[i];
+          const double v[9] = {(double)d.electronID, (double)d.primary, (double)d.trajStep, d.energyeV,
+                               d.position[0], d.position[1], d.position[2], d.theta, d.phi};
+          for (int k = 0; k < 9; ++k) r(i, k) = v[k];
+        }
+        return a;
+      });
 
   py::class_<monsel_job_stats>(m, "JobStats")
       .def_readonly("primaries", &monsel_job_stats::primaries)

@@ -24,7 +24,9 @@ implicit and named "chamber".
    "beam": {"energy_eV": 500, "sigma_nm": 0.5, "center_nm": [0, 0, -20]},
    "ref_compat": "reference"}
 
-Shape kinds: NormalMultiPlaneShape (planes), NormalCylindricalShape (end0_nm,
+A material may give its core levels as "core_energies_eV" or ask for
+SEmaterial::setEstimatedCoreEnergy with "estimated_core_energy_eV" (the
+cutoff).  Shape kinds: NormalMultiPlaneShape (planes), NormalCylindricalShape (end0_nm,
 end1_nm, radius_nm), line (NShapes::createLine: topz_nm, width_nm,
 length_nm, thetal_deg, thetar_deg, radl_nm, radr_nm, reference_compat,
 translate_nm), step_edge (NShapes::createStepEdge: z_nm, step_nm),
@@ -71,6 +73,8 @@ def _material(spec):
         mat.setEnergyCBbottom(M.ToSI.eV(spec["cb_bottom_eV"]))
     if "core_energies_eV" in spec:
         mat.setCoreEnergy([M.ToSI.eV(v) for v in spec["core_energies_eV"]])
+    if "estimated_core_energy_eV" in spec:   # SEmaterial::setEstimatedCoreEnergy(cutoff)
+        mat.setEstimatedCoreEnergy(M.ToSI.eV(float(spec["estimated_core_energy_eV"])))
     return comp, mat
 
 

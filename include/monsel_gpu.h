@@ -236,6 +236,8 @@ typedef struct monsel_job_desc {
   int32_t max_steps;      /* per-electron watchdog; <= 0 selects 1000000 */
   double se_cutoff_ev;    /* SE yield = back-escapes in bins j < (int)(cutoff / binwidth) */
   uint64_t primary_offset; /* first primary index (for splitting one job) */
+  int64_t log_detected;    /* BackscatterStats::setLogDetected (BackscatterStats.cu:166-168): keep up to this
+                            * many detected-electron records (monsel_job_read_detected); 0 = none */
 } monsel_job_desc;
 
 /* Caller-owned result buffers; any pointer may be NULL.  Histogram slot
@@ -252,6 +254,18 @@ typedef struct monsel_tallies {
   uint64_t* elevation_hist;   /* [180 + 2] over [0, pi] */
   uint64_t* azimuth_hist;     /* [360 + 2] over [0, 2 pi] */
 } monsel_tallies;
+
+/* One detected electron (BackscatterStats::Datum, BackscatterStats.cu:40-48,
+ * logged at :166-168): which electron, its step count, exit energy, the
+ * point where it reached the chamber wall and its direction there. */
+typedef struct monsel_detected {
+  uint64_t electron;     /* lineage of the electron: unique within its primary's cascade */
+  uint32_t primary;      /* primary index (the cascade it belongs to) */
+  uint32_t step;         /* Electron::getStepCount at detection */
+  double energy_ev;      /* exit energy (eV) */
+  double position[3];    /* the tally point on the chamber sphere (m) */
+  double theta, phi;     /* direction: polar angle from +z, azimuth (Electron::getTheta/getPhi) */
+} monsel_detected;
 
 typedef struct monsel_job_stats {
   uint64_t primaries;
@@ -319,6 +333,12 @@ int monsel_job_wait(monsel_ctx ctx, monsel_job_id job, int64_t timeout_us);
 int monsel_job_read(monsel_ctx ctx, monsel_job_id job, monsel_tallies* out);
 int monsel_job_stats_get(monsel_ctx ctx, monsel_job_id job, monsel_job_stats* out);
 int monsel_job_release(monsel_ctx ctx, monsel_job_id job);
+/* Detected-electron records of a completed job that asked for them
+ * (monsel_job_desc.log_detected > 0), in no particular order: copies up to
+ * max of them into out and sets *n_detected to the number the job detected
+ * (more than it kept when its log_detected was too small). */
+int monsel_job_read_detected(monsel_ctx ctx, monsel_job_id job, monsel_detected* out, int64_t max,
+                             int64_t* n_detected);
 /* Ends a job early.  No reference counterpart (runMultipleTrajectories,
  * MonteCarloSS.cu:222-231, always runs to its end); a serving loop uses it to
  * drop a request that timed out.  A job still waiting for admission ends at once; a running one

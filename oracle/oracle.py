@@ -30,6 +30,7 @@ class JobDesc(ctypes.Structure):
         ("max_steps", ctypes.c_int32),
         ("se_cutoff_ev", ctypes.c_double),
         ("primary_offset", ctypes.c_uint64),
+        ("log_detected", ctypes.c_int64),
     ]
 
 
