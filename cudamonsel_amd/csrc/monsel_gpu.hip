@@ -317,8 +317,8 @@ __device__ __forceinline__ int barrier_core(const DevScene& S, const MONSEL_CS D
   float rootPerpE = 0.f, rootDiff = 0.f;
   if (perpE == 0.f || perpE <= deltaU) transmits = false;
   else {
-    rootPerpE = __builtin_sqrtf(perpE);
-    rootDiff = __builtin_sqrtf(perpE - deltaU);
+    rootPerpE = fsqrt(perpE);
+    rootDiff = fsqrt(perpE - deltaU);
     if (msm.barrier_kind == MONSEL_BARRIER_EXPQM_CLASSICAL) transmits = true;
     else {
       float tp;
@@ -391,7 +391,7 @@ __device__ __forceinline__ void set_anchor(float x, float y, float z) {
 __device__ __forceinline__ float from_anchor(float x, float y, float z) {
   const float* a = lane_anchor();
   const float dx = x - a[0], dy = y - a[kLaneBlock], dz = z - a[2 * kLaneBlock];
-  return __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
+  return fsqrt(dx * dx + dy * dy + dz * dz);
 }
 
 /* What the start of a step leaves for its end. */
@@ -445,7 +445,7 @@ __device__ __forceinline__ void begin_step_in(const DevScene& S, int region, Ele
     }
     PT_END(PH_RATE_LOOP, t_rl);
     if (total != 0.f) {
-      float fp = -__logf(u[0]) / total;
+      float fp = -flog(u[0]) / total;
       freePath = fp > maxFreePath ? maxFreePath : fp;
     } else {
       freePath = maxFreePath;
@@ -665,7 +665,7 @@ __device__ __forceinline__ void make_primary(const DevScene& S, const JobRec& J,
   uint32_t a0, a1, a2, a3;
   philox4x32_10(kPrimaryBlock, e.lin_lo, e.lin_hi, e.primary, kPhiloxKey0, kPhiloxKey1, a0, a1, a2, a3);
   const float u1 = u01(a0), u2 = u01(a1);
-  float r = __builtin_sqrtf(-2.f * __logf(u1)) * J.width;
+  float r = fsqrt(-2.f * flog(u1)) * J.width;
   float s, c;
   __sincosf(kTwoPI * u2, &s, &c);
   const double* cen = J.centers + 3 * (size_t)pix;

@@ -40,6 +40,16 @@ constexpr float kEcharge = 1.60217653e-19f;         /* J per eV */
 
 /* ----------------------------------------------------------- RNG ------- */
 
+/* Square root, natural log and exponential straight from the hardware
+ * instructions (v_sqrt_f32, v_log_f32, v_exp_f32: about 1 ulp; FP32
+ * denormals are flushed in this code, so no rescaling is needed).  The
+ * compiler's own expansions refine sqrt to correct rounding with a
+ * reciprocal-root Newton step and scale log/exp around denormal ranges, some
+ * ten instructions each, on the transport loop's critical path. */
+MD float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+MD float flog(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
+MD float fexp(float x) { return __builtin_amdgcn_exp2f(x * 1.442695040888963407f); }
+
 /* Philox4x32-10 (Salmon et al., SC'11).  Counter words: (block, lineage lo,
  * lineage hi, primary index) under one fixed key; a job's seed enters
  * through the lineage of its primaries (splitmix64 of the seed), a
@@ -139,7 +149,7 @@ MD void rotate_dir(float& dx, float& dy, float& dz, float ca, float sa, float cb
   float st2 = dx * dx + dy * dy;
   float cp = 1.f, sp = 0.f, st = 0.f;
   if (st2 > 0.f) {
-    st = __builtin_sqrtf(st2);
+    st = fsqrt(st2);
     float inv = 1.f / st;
     cp = dx * inv;
     sp = dy * inv;
@@ -194,7 +204,7 @@ MD int u_index0(float reduced, int len) {
 /* BrowningEmpiricalCrossSection.cu:24-29 (nm^2) */
 MD float browning_sigma(const MONSEL_CS DElem& el, float EeV) {
   float e = EeV * 1.0e-3f;
-  float re = __builtin_sqrtf(e);
+  float re = fsqrt(e);
   return 3.0e-4f * el.zp17 / (e + 0.005f * el.zp17 * re + 0.0007f * el.zp2 / re);
 }
 
@@ -249,7 +259,7 @@ MD void stage_mott_sigma(const DevScene& S) {
 
 /* NISTMottRS::totalCrossSection (NISTMottRS.cu:103-119), nm^2 */
 MD float mott_table_sigma(const DevScene& S, int mott, float EeV) {
-  float red = (__logf(EeV) - kMottLn50) * kMottInvParam;
+  float red = (flog(EeV) - kMottLn50) * kMottInvParam;
   int i0 = u_index0(red, kMottNE);
   float f0, f1, f2, f3;
   if (mott < kLdsMott) {
@@ -283,7 +293,7 @@ MD float elem_sigma(const DevScene& S, const MONSEL_CS DElem& el, float EeV) {
  * over (ln E, U) (NISTMottRS.cu:134-150, ULagrangeInterpolation.cu:55-78);
  * returns cos(theta) clamped as the reference does. */
 MD float mott_cos(const DevScene& S, int mott, float EeV, float u) {
-  float rx = (__logf(EeV) - kMottLn50) * kMottInvParam;
+  float rx = (flog(EeV) - kMottLn50) * kMottInvParam;
   int i0 = u_index0(rx, kMottNE);
   float ry = u * 200.0f;
   int j0 = u_index0(ry, kMottNQ);
@@ -326,11 +336,11 @@ MD float csd_compute(const MONSEL_CS DCSD& c, float len, float kE) {
   if (!c.active || c.nce == 0) return 0.f;
   if (kE < c.minE || kE <= 0.f) return 0.f;
   if (kE <= c.breakE) {
-    const float eps = 1.5f * c.gamma * len * kE * __builtin_sqrtf(kE);
+    const float eps = 1.5f * c.gamma * len * kE * fsqrt(kE);
     return kE * nieminen_factor(eps);
   }
   float s = 0.f;
-  for (int i = 0; i < c.nce; ++i) s += c.coef[i] * __logf(c.recipJ[i] * kE + c.beta[i]);
+  for (int i = 0; i < c.nce; ++i) s += c.coef[i] * flog(c.recipJ[i] * kE + c.beta[i]);
   const float loss = s * len / kE;
   if (!(loss > 0.1f * kE)) return -loss;
   return csd_compute_split(c, len, kE);
@@ -345,11 +355,11 @@ MD float csd_compute_split(const MONSEL_CS DCSD& c, float len, float kE) {
     float dE = 0.f;
     if (!(E < c.minE || E <= 0.f)) {
       if (E <= c.breakE) {
-        const float eps = 1.5f * c.gamma * seg * E * __builtin_sqrtf(E);
+        const float eps = 1.5f * c.gamma * seg * E * fsqrt(E);
         dE = E * nieminen_factor(eps);
       } else {
         float s = 0.f;
-        for (int i = 0; i < c.nce; ++i) s += c.coef[i] * __logf(c.recipJ[i] * E + c.beta[i]);
+        for (int i = 0; i < c.nce; ++i) s += c.coef[i] * flog(c.recipJ[i] * E + c.beta[i]);
         const float loss = s * seg / E;
         if (loss > 0.1f * E && d < 30) {
           pending |= 1u << (d + 1);
@@ -572,14 +582,14 @@ MD float mech_rate(const DevScene& S, const MONSEL_CS DMech& m, float EeV) {
       return -csd_compute(m.csd, 0.1f, EeV) * 10.f * m.p[2];
     case MONSEL_MECH_POLARON_TRAP: /* GanachaudMokraniPolaronTrapSM.cu:21-27 */
       if (EeV > m.p[2]) return 0.f;
-      return m.p[0] * __expf(-m.p[1] * EeV);
+      return m.p[0] * fexp(-m.p[1] * EeV);
     case MONSEL_MECH_PHONON: { /* GanachaudMokraniPhononInelasticSM.cu:46-72 */
       if (EeV < m.p[0]) return 0.f;
       float x = m.p[0] / EeV;
-      if (x < 0.1f) return m.p[1] * x * __logf(4.f / x);
+      if (x < 0.1f) return m.p[1] * x * flog(4.f / x);
       if (x >= 1.f) return 0.f;
-      float temp = __builtin_sqrtf(1.f - x);
-      return m.p[1] * x * __logf((1.f + temp) / (1.f - temp));
+      float temp = fsqrt(1.f - x);
+      return m.p[1] * x * flog((1.f + temp) / (1.f - temp));
     }
     case MONSEL_MECH_TABULATED_INELASTIC: { /* TabulatedInelasticSM.cu:436-459 */
       float kE = (EeV + m.p[TP_OFFSET]) * kEcharge;
@@ -645,7 +655,7 @@ MD double pick_be(const MONSEL_CS DMech& m, double Eq, double deltaE) {
 MD void isotropic_dir(float u1, float u2, float& dx, float& dy, float& dz) {
   /* theta = acos(1 - 2 u1), phi = 2 pi u2 */
   float ct = 1.f - 2.f * u1;
-  float st = __builtin_sqrtf(fmaxf(0.f, 1.f - ct * ct));
+  float st = fsqrt(fmaxf(0.f, 1.f - ct * ct));
   float sp, cp;
   __sincosf(kTwoPI * u2, &sp, &cp);
   dx = st * cp;
@@ -748,7 +758,7 @@ MD Spawn tabulated_scatter(const DevScene& S, const MONSEL_CS DMech& m, Elec& e,
         __sincosf(kTwoPI * u[6], &sp2, &cp2);
         sincosf(thSE, &st2, &ct2);
         rotate_dir(wx, wy, wz, ct2, st2, cp2, sp2);
-        float wst = __builtin_sqrtf(fmaxf(0.f, wx * wx + wy * wy));
+        float wst = fsqrt(fmaxf(0.f, wx * wx + wy * wy));
         float wcp = 1.f, wsp = 0.f;
         if (wst > 0.f) {
           wcp = wx / wst;
@@ -813,7 +823,7 @@ MD Spawn mech_scatter(const DevScene& S, const MONSEL_CS DMech& m, Elec& e, floa
         ub = u[4];
       }
       ca = fminf(1.f, fmaxf(-1.f, ca));
-      float sa = __builtin_sqrtf(fmaxf(0.f, 1.f - ca * ca));
+      float sa = fsqrt(fmaxf(0.f, 1.f - ca * ca));
       float sb, cb;
       __sincosf(kTwoPI * ub, &sb, &cb);
       rotate_dir(e.dx, e.dy, e.dz, ca, sa, cb, sb);
@@ -849,12 +859,12 @@ MD Spawn mech_scatter(const DevScene& S, const MONSEL_CS DMech& m, Elec& e, floa
       float ct;
       if (x < 0.1f) ct = 1.f + ((x * x) - (exp2f(4.f * r0) * powf(x, 2.f - 2.f * r0))) / 8.f;
       else {
-        float root = __builtin_sqrtf(1.f - x);
+        float root = fsqrt(1.f - x);
         float temp = powf(((-2.f * (1.f + root)) + x) / ((-2.f * (1.f - root)) + x), r0);
         ct = temp + (((x - 2.f) * (temp - 1.f)) / 2.f / root);
       }
       ct = fminf(1.f, fmaxf(-1.f, ct));
-      float st = __builtin_sqrtf(fmaxf(0.f, 1.f - ct * ct));
+      float st = fsqrt(fmaxf(0.f, 1.f - ct * ct));
       float sb, cb;
       __sincosf(kTwoPI * r1, &sb, &cb);
       rotate_dir(e.dx, e.dy, e.dz, ct, st, cb, sb);
@@ -960,7 +970,7 @@ MD Interval prim_interval(const DevScene& S, const MONSEL_CS DPrim& pr, float px
         iv.tout = -kInf;
         return iv;
       }
-      float sq = __builtin_sqrtf(disc);
+      float sq = fsqrt(disc);
       float q = -(B + (B >= 0.f ? sq : -sq));
       float ra = q / A, rb = (q != 0.f) ? C / q : ra;
       rin = fminf(ra, rb);
@@ -986,7 +996,7 @@ MD Interval prim_interval(const DevScene& S, const MONSEL_CS DPrim& pr, float px
     iv.tout = -kInf;
     return iv;
   }
-  float sq = __builtin_sqrtf(disc);
+  float sq = fsqrt(disc);
   iv.tin = -B - sq;
   iv.tout = -B + sq;
   iv.fin = iv.fout = -3;
@@ -1061,10 +1071,10 @@ MD float prim_gmax(const DevScene& S, const MONSEL_CS DPrim& pr, float px, float
   if (pr.kind == PRIM_CYLINDER) {
     const float apar = wx * pr.ax[0] + wy * pr.ax[1] + wz * pr.ax[2];
     const float qx = wx - apar * pr.ax[0], qy = wy - apar * pr.ax[1], qz = wz - apar * pr.ax[2];
-    const float radial = __builtin_sqrtf(qx * qx + qy * qy + qz * qz) - __builtin_sqrtf(pr.r2);
+    const float radial = fsqrt(qx * qx + qy * qy + qz * qz) - fsqrt(pr.r2);
     return fmaxf(radial, fmaxf(-apar, apar - pr.len));
   }
-  return __builtin_sqrtf(wx * wx + wy * wy + wz * wz) - __builtin_sqrtf(pr.r2);
+  return fsqrt(wx * wx + wy * wy + wz * wz) - fsqrt(pr.r2);
 }
 
 MD bool truth(const MONSEL_CS DShape& sh, uint32_t bits) { return (sh.truth[bits >> 5] >> (bits & 31u)) & 1u; }
@@ -1088,16 +1098,16 @@ MD float prim_gmax_in(const DevScene& S, const MONSEL_CS DPrim& pr, float px, fl
     return g * pr.sdscale;
   }
   const float wx = px - pr.e0[0], wy = py - pr.e0[1], wz = pz - pr.e0[2];
-  const float r = __builtin_sqrtf(pr.r2);
+  const float r = fsqrt(pr.r2);
   float g, scale;
   if (pr.kind == PRIM_CYLINDER) {
     const float apar = wx * pr.ax[0] + wy * pr.ax[1] + wz * pr.ax[2];
     const float qx = wx - apar * pr.ax[0], qy = wy - apar * pr.ax[1], qz = wz - apar * pr.ax[2];
-    const float q = __builtin_sqrtf(qx * qx + qy * qy + qz * qz);
+    const float q = fsqrt(qx * qx + qy * qy + qz * qz);
     g = fmaxf(q - r, fmaxf(-apar, apar - pr.len));
     scale = q + fabsf(apar) + pr.len + r;
   } else {
-    const float w = __builtin_sqrtf(wx * wx + wy * wy + wz * wz);
+    const float w = fsqrt(wx * wx + wy * wy + wz * wz);
     g = w - r;
     scale = w + r;
   }
