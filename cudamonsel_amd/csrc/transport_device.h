@@ -857,10 +857,13 @@ MD Spawn mech_scatter(const DevScene& S, const MONSEL_CS DMech& m, Elec& e, floa
       float x = m.p[0] / kE0;
       const float r0 = u[2], r1 = u[3];
       float ct;
-      if (x < 0.1f) ct = 1.f + ((x * x) - (exp2f(4.f * r0) * powf(x, 2.f - 2.f * r0))) / 8.f;
+      /* x^a = 2^(a log2 x) for x > 0, on the hardware exp2/log2 */
+      if (x < 0.1f)
+        ct = 1.f + ((x * x) - __builtin_amdgcn_exp2f(4.f * r0 + (2.f - 2.f * r0) * __builtin_amdgcn_logf(x))) / 8.f;
       else {
         float root = fsqrt(1.f - x);
-        float temp = powf(((-2.f * (1.f + root)) + x) / ((-2.f * (1.f - root)) + x), r0);
+        float temp = __builtin_amdgcn_exp2f(
+            r0 * __builtin_amdgcn_logf(((-2.f * (1.f + root)) + x) / ((-2.f * (1.f - root)) + x)));
         ct = temp + (((x - 2.f) * (temp - 1.f)) / 2.f / root);
       }
       ct = fminf(1.f, fmaxf(-1.f, ct));
@@ -972,7 +975,7 @@ MD Interval prim_interval(const DevScene& S, const MONSEL_CS DPrim& pr, float px
       }
       float sq = fsqrt(disc);
       float q = -(B + (B >= 0.f ? sq : -sq));
-      float ra = q / A, rb = (q != 0.f) ? C / q : ra;
+      float ra = q / A, rb = (q != 0.f) ? C * __builtin_amdgcn_rcpf(q) : ra;
       rin = fminf(ra, rb);
       rout = fmaxf(ra, rb);
     } else if (C > 0.f) {
