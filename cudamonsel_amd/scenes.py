@@ -7,10 +7,11 @@ helpers take nm / eV arguments and convert.
 
 Si and Cu: the reference reads Si's TabulatedInelasticSM tables from files
 that are not shipped (LinesOnLayers.cu:187-193) and defines no Cu model.  The
-default models here use the reference's FittedInelSM + JoyLuoNieminenCSD
-mechanisms with the parameters below (a stand-in, stated in every report);
-si_tabulated() builds the reference's TabulatedInelasticSM form on tables
-generated by tables.py.
+"Si" and "Cu" models here use the reference's FittedInelSM +
+JoyLuoNieminenCSD mechanisms with the parameters below (a stand-in, stated in
+every report); "Si-dielectric" and "Cu-dielectric" build the reference's
+TabulatedInelasticSM form on tables computed from a dielectric model
+(dielectric.py); si_tabulated() takes any four table files.
 """
 import math
 
@@ -153,7 +154,37 @@ def cu(sc):
     return msm
 
 
-MATERIALS = {"PMMA": pmma, "Si": si, "Cu": cu}
+def si_dielectric(sc, method_se=3):
+    """Si as LinesOnLayers.cu:195-212 -- NIST Mott elastic, TabulatedInelasticSM
+    (method 3, energy offset 13.54 eV) and the phonon mechanism -- on tables
+    computed from a dielectric model (dielectric.py) in place of the
+    reference's unshipped files."""
+    from . import dielectric
+    paths, offset = dielectric.make_tables("Si")
+    return si_tabulated(sc, paths, energy_offset_eV=offset, method_se=method_se)
+
+
+def cu_dielectric(sc, method_se=3):
+    """Cu: NIST Mott elastic and TabulatedInelasticSM on dielectric-model
+    tables (dielectric.py)."""
+    from . import dielectric
+    paths, offset = dielectric.make_tables("Cu")
+    density, workfun = 8960.0, 4.65
+    mat = M.SEmaterial([M.Elements.Cu], [1.0], density, "Copper")
+    mat.setWorkfunction(M.ToSI.eV(workfun))
+    mat.setEnergyCBbottom(M.ToSI.eV(-workfun - 7.0))
+    mat.setEstimatedCoreEnergy()
+    el = M.SelectableElasticSM(mat, M.NISTMottRS.Factory)
+    tab = M.TabulatedInelasticSM(mat, method_se, list(paths), M.ToSI.eV(offset))
+    bar = M.ExpQMBarrierSM(mat)
+    msm = M.MONSEL_MaterialScatterModel(mat, bar)
+    msm.addScatterMechanism(el)
+    msm.addScatterMechanism(tab)
+    sc.hold(mat, el, tab, bar, msm)
+    return msm
+
+
+MATERIALS = {"PMMA": pmma, "Si": si, "Cu": cu, "Si-dielectric": si_dielectric, "Cu-dielectric": cu_dielectric}
 
 
 def _chamber(sc):
