@@ -2048,7 +2048,12 @@ struct Job {
   uint64_t reserved = 0;      /* queue entries reserved for its electrons in flight */
   uint32_t outstanding = 0;   /* its electrons in the current queue */
   int generations = 0;
-  DBuf prim_counts, prim_fwd, prim_energy, px, hist, stats_dev, centers_dev, wd, det;
+  /* its tallies and pixel centres in its slot's buffer (monsel_ctx_s::slotbuf):
+   * [stats | histograms | per-pixel sums | per-primary counts] cleared at
+   * admission, then the centres */
+  size_t o_hist = 0, o_px = 0, o_pc = 0, o_pf = 0, o_pe = 0, o_cen = 0, zero_bytes = 0;
+  char* base = nullptr;
+  DBuf wd, det;
   std::vector<monsel_detected> detected; /* the kept records, after completion */
   uint64_t n_detected = 0;
 };
@@ -2083,6 +2088,10 @@ struct monsel_ctx_s {
   std::vector<std::shared_ptr<Job>> active;
   std::vector<int> free_slots;
   JobRec jobrec[kMaxJobs];
+  /* device memory of each job slot, kept from job to job (grown as needed,
+   * given back when a job left it above kSlotKeepBytes): admitting a job
+   * then allocates nothing */
+  DBuf slotbuf[kMaxJobs];
   SceneDev batch;             /* the scenes of the active jobs, side by side */
   std::vector<std::shared_ptr<SceneDev>> batch_parts;
   std::vector<int> batch_roots;
@@ -2173,27 +2182,35 @@ void setup_job(monsel_ctx_s* C, Job& job) {
   job.slot = C->free_slots.back();
   C->free_slots.pop_back();
   const size_t n = (size_t)std::max<uint64_t>(job.total, 1);
-  job.prim_counts.ensure(n * 4);
-  job.prim_fwd.ensure(n * 4);
-  job.prim_energy.ensure(n * 8);
-  job.px.ensure((size_t)np * 6 * 8);
-  job.hist.ensure((size_t)hist_len(nbins) * 8);
-  job.stats_dev.ensure(8 * 8 * kStatStripes);
-  job.centers_dev.ensure((size_t)np * 3 * 8);
-  HIP_TRY(hipMemsetAsync(job.prim_counts.p, 0, n * 4, C->stream));
-  HIP_TRY(hipMemsetAsync(job.prim_fwd.p, 0, n * 4, C->stream));
-  HIP_TRY(hipMemsetAsync(job.prim_energy.p, 0, n * 8, C->stream));
-  HIP_TRY(hipMemsetAsync(job.px.p, 0, (size_t)np * 6 * 8, C->stream));
-  HIP_TRY(hipMemsetAsync(job.hist.p, 0, (size_t)hist_len(nbins) * 8, C->stream));
-  HIP_TRY(hipMemsetAsync(job.stats_dev.p, 0, 8 * 8 * kStatStripes, C->stream));
-  HIP_TRY(hipMemcpyAsync(job.centers_dev.p, job.centers.data(), (size_t)np * 3 * 8, hipMemcpyHostToDevice, C->stream));
+  /* one buffer per slot: stats, histograms and per-pixel sums first (read
+   * back together), then the per-primary counts, all cleared by one fill,
+   * then the pixel centres */
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off += (bytes + 255) & ~(size_t)255;
+    return o;
+  };
+  take(8 * 8 * kStatStripes);
+  job.o_hist = take((size_t)hist_len(nbins) * 8);
+  job.o_px = take((size_t)np * 6 * 8);
+  job.o_pc = take(n * 4);
+  job.o_pf = take(n * 4);
+  job.o_pe = take(n * 8);
+  job.zero_bytes = off;
+  job.o_cen = take((size_t)np * 3 * 8);
+  C->slotbuf[job.slot].ensure(off);
+  job.base = C->slotbuf[job.slot].as<char>();
+  HIP_TRY(hipMemsetAsync(job.base, 0, job.zero_bytes, C->stream));
+  HIP_TRY(hipMemcpyAsync(job.base + job.o_cen, job.centers.data(), (size_t)np * 3 * 8, hipMemcpyHostToDevice,
+                         C->stream));
   const HostScene& H = job.sd->host;
   JobRec& R = C->jobrec[job.slot];
   memset(&R, 0, sizeof(R));
   R.lin0 = job_lineage(jd.seed);
   R.E0 = (float)E0eV;
   R.width = (float)(jd.beam_width * 1e9);
-  R.centers = job.centers_dev.as<double>();
+  R.centers = (const double*)(job.base + job.o_cen);
   R.n_pixels = np;
   R.ppp = jd.primaries_per_pixel;
   R.prim_base = (uint32_t)jd.primary_offset;
@@ -2205,11 +2222,11 @@ void setup_job(monsel_ctx_s* C, Job& job) {
   R.E0eV = E0eV;
   for (int k = 0; k < 3; ++k) R.chamber_c[k] = H.chamber_c[k];
   R.chamber_r = H.chamber_r;
-  R.prim_counts = job.prim_counts.as<uint32_t>();
-  R.prim_fwd = job.prim_fwd.as<uint32_t>();
-  R.prim_energy = job.prim_energy.as<unsigned long long>();
-  R.hist = job.hist.as<unsigned long long>();
-  R.stats = job.stats_dev.as<unsigned long long>();
+  R.prim_counts = (uint32_t*)(job.base + job.o_pc);
+  R.prim_fwd = (uint32_t*)(job.base + job.o_pf);
+  R.prim_energy = (unsigned long long*)(job.base + job.o_pe);
+  R.hist = (unsigned long long*)(job.base + job.o_hist);
+  R.stats = (unsigned long long*)job.base;
   /* MONSEL_DEBUG_WATCHDOG=1: keep the state of the first watchdog drops and
    * print it when the job completes */
   if (getenv("MONSEL_DEBUG_WATCHDOG")) {
@@ -2226,8 +2243,10 @@ void setup_job(monsel_ctx_s* C, Job& job) {
     R.det_log = (monsel_detected*)(job.det.as<char>() + 16);
     R.det_max = (unsigned long long)jd.log_detected;
   }
+  /* the copies' sources (the job's centres, its record in C->jobrec) live
+   * until the job ends, and the next launch orders after them on the
+   * stream: no synchronisation here */
   HIP_TRY(hipMemcpyAsync(C->jobtab.as<JobRec>() + job.slot, &R, sizeof(JobRec), hipMemcpyHostToDevice, C->stream));
-  HIP_TRY(hipStreamSynchronize(C->stream));
 }
 
 /* Per-pixel reduction and read-back of a job whose electrons are all done. */
@@ -2240,14 +2259,16 @@ void complete_job(monsel_ctx_s* C, Job& job) {
     g_hip_call = "launch k_reduce_primaries";
     hipLaunchKernelGGL(k_reduce_primaries, dim3((unsigned)((job.total + rb - 1) / rb)), dim3(rb), 0, C->stream,
                        (const JobRec*)C->jobtab.as<JobRec>(), job.slot, (uint32_t)job.total,
-                       job.px.as<unsigned long long>());
+                       (unsigned long long*)(job.base + job.o_px));
     HIP_TRY(hipGetLastError());
   }
-  std::vector<unsigned long long> px((size_t)np * 6), h((size_t)hist_len(nbins)), sts(8 * kStatStripes), st(8, 0ull);
-  HIP_TRY(hipMemcpyAsync(px.data(), job.px.p, px.size() * 8, hipMemcpyDeviceToHost, C->stream));
-  HIP_TRY(hipMemcpyAsync(h.data(), job.hist.p, h.size() * 8, hipMemcpyDeviceToHost, C->stream));
-  HIP_TRY(hipMemcpyAsync(sts.data(), job.stats_dev.p, 8 * 8 * kStatStripes, hipMemcpyDeviceToHost, C->stream));
+  /* stats, histograms and per-pixel sums in one read-back */
+  std::vector<unsigned long long> all((job.o_px + (size_t)np * 6 * 8) / 8), st(8, 0ull);
+  HIP_TRY(hipMemcpyAsync(all.data(), job.base, all.size() * 8, hipMemcpyDeviceToHost, C->stream));
   HIP_TRY(hipStreamSynchronize(C->stream));
+  const unsigned long long* sts = all.data();
+  const std::vector<unsigned long long> h(all.begin() + job.o_hist / 8, all.begin() + job.o_hist / 8 + hist_len(nbins));
+  const std::vector<unsigned long long> px(all.begin() + job.o_px / 8, all.begin() + job.o_px / 8 + (size_t)np * 6);
   for (int k = 0; k < 8 * kStatStripes; ++k) st[k % 8] += sts[k];
   job.back.assign(px.begin(), px.begin() + np);
   job.fwd.assign(px.begin() + np, px.begin() + 2 * np);
@@ -2302,16 +2323,14 @@ void complete_job(monsel_ctx_s* C, Job& job) {
   }
 }
 
+/* a slot keeps its buffer for the next job up to this size */
+constexpr size_t kSlotKeepBytes = (size_t)64 << 20;
+
 void release_job(monsel_ctx_s* C, Job& job) {
+  if (job.slot >= 0 && C->slotbuf[job.slot].n > kSlotKeepBytes) C->slotbuf[job.slot].reset();
   C->free_slots.push_back(job.slot);
   job.slot = -1;
-  job.prim_counts.reset();
-  job.prim_fwd.reset();
-  job.prim_energy.reset();
-  job.px.reset();
-  job.hist.reset();
-  job.stats_dev.reset();
-  job.centers_dev.reset();
+  job.base = nullptr;
   job.wd.reset();
   job.det.reset();
   job.sd.reset();
