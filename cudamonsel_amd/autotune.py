@@ -1,11 +1,18 @@
 """Launch-shape autotuner for the batched form.
 
-Coordinate search over the engine's tuning knobs -- workgroup size, regroup
+Coordinate descent over the engine's tuning knobs -- workgroup size, regroup
 interval (steps per launch), queue depth (new primaries admitted per launch),
-jobs in flight, tail-compaction threshold, secondary handoff and the lanes a
-wave gathers for findEndOfStep -- timing a
-caller-supplied workload on a fresh context for each candidate.  The winner can be stored as JSON and passed to
-`abi.Engine(**tuning)`.
+jobs in flight, tail-compaction threshold, secondary handoff, the lanes a
+wave gathers for findEndOfStep, the between-launch queue sort and the
+admission order -- timing a caller-supplied workload on a fresh context for
+each candidate (best of `repeats` runs).  Sweeps repeat from the current best
+until one changes nothing, so that a knob whose best value depends on
+another's is revisited after that one moved; a candidate replaces the best
+only when it is faster by more than `min_gain` (the run-to-run noise).  The
+winner can be stored as JSON and passed to `abi.Engine(**tuning)`.
+
+Usage: python -m cudamonsel_amd.autotune batch [out.json]
+       (the 512-job batch workload of serving.batch_workload)
 """
 import json
 import os
@@ -21,37 +28,57 @@ SPACE = {
     "tail_requeue": [0, 16, 48, -1],
     "spawn_handoff": [0, -1],
     "geometry_batch": [0, 4, 8],
+    "sort_queue": [0, -1],
+    "admission_order": [0, 1],
 }
 
 
-def _time(workload, tuning, device):
+def _time(workload, tuning, device, repeats=1):
     with Engine(device, **tuning) as eng:
-        t0 = time.perf_counter()
-        workload(eng)
-        return time.perf_counter() - t0
+        best = float("inf")
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            workload(eng)
+            best = min(best, time.perf_counter() - t0)
+        return best
 
 
-def autotune(workload, space=None, device=0, log=None):
+def autotune(workload, space=None, device=0, log=None, repeats=2, min_gain=0.01, max_sweeps=4):
     """workload(engine) runs the jobs to be tuned for.  Returns (best tuning
-    dict, [(tuning, seconds), ...]) from a one-knob-at-a-time search that
-    starts at the defaults (all zeros)."""
+    dict, [(tuning, seconds), ...]) from coordinate descent that starts at
+    the defaults (the first value of every knob) and sweeps all knobs until a
+    sweep changes nothing (at most max_sweeps)."""
     space = space or SPACE
     best = {k: v[0] for k, v in space.items()}
     trials = []
-    best_t = _time(workload, best, device)
-    trials.append((dict(best), best_t))
+    seen = {}
+
+    def timed(cand):
+        key = tuple(sorted(cand.items()))
+        if key not in seen:
+            seen[key] = _time(workload, cand, device, repeats)
+            trials.append((dict(cand), seen[key]))
+        return seen[key]
+
+    best_t = timed(best)
     if log:
         log(f"autotune start {best} {best_t:.3f}s")
-    for knob, values in space.items():
-        for v in values[1:]:
-            cand = dict(best)
-            cand[knob] = v
-            t = _time(workload, cand, device)
-            trials.append((cand, t))
-            if log:
-                log(f"autotune {knob}={v}: {t:.3f}s")
-            if t < best_t:
-                best, best_t = cand, t
+    for sweep in range(max_sweeps):
+        changed = False
+        for knob, values in space.items():
+            for v in values:
+                if v == best[knob]:
+                    continue
+                cand = dict(best)
+                cand[knob] = v
+                t = timed(cand)
+                if log:
+                    log(f"autotune sweep {sweep} {knob}={v}: {t:.3f}s (best {best_t:.3f}s)")
+                if t < best_t * (1.0 - min_gain):
+                    best, best_t = cand, t
+                    changed = True
+        if not changed:
+            break
     return best, trials
 
 
@@ -64,3 +91,23 @@ def save(path, tuning, trials):
 def load(path):
     with open(path) as f:
         return json.load(f)["tuning"]
+
+
+def _batch_main(out):
+    import sys
+    from . import serving
+    jobs = serving.batch_workload(512, seed=4)
+
+    def workload(eng):
+        serving.run_batch(eng, jobs)
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    best, trials = autotune(workload, log=log)
+    save(out, {k: v for k, v in best.items()}, trials)
+    print(json.dumps(best))
+
+
+if __name__ == "__main__":
+    import sys
+    if len(sys.argv) > 1 and sys.argv[1] == "batch":
+        here = os.path.dirname(os.path.abspath(__file__))
+        _batch_main(sys.argv[2] if len(sys.argv) > 2 else os.path.join(here, "data", "tuning_batch.json"))
