@@ -1448,6 +1448,37 @@ struct DBuf {
   }
 };
 
+/* pinned host buffer: copies to and from it run on the stream without the
+ * host waiting (pageable copies are staged and wait) */
+struct HBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  HBuf() = default;
+  HBuf(const HBuf&) = delete;
+  HBuf& operator=(const HBuf&) = delete;
+  ~HBuf() { reset(); }
+  void reset() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  /* grows without keeping the contents; only when no copy uses it */
+  void ensure(size_t bytes) {
+    if (bytes <= n) return;
+    reset();
+    hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw MonselError(MONSEL_E_NOMEM, "hipHostMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    }
+    n = bytes;
+  }
+  template <class T>
+  T* as() const {
+    return (T*)p;
+  }
+};
+
 struct HostScene {
   std::vector<DRegion> regions;
   std::vector<int> children;
@@ -2085,9 +2116,17 @@ struct monsel_ctx_s {
   double transport_ms = 0.;
   unsigned int* host_count = nullptr; /* pinned: out, esc, per-job counts */
   uint32_t* host_segs = nullptr;      /* pinned: segment staging */
+  HBuf pin_count, pin_segs;
   std::vector<std::shared_ptr<Job>> active;
   std::vector<int> free_slots;
-  JobRec jobrec[kMaxJobs];
+  /* host copy of the device job table (pinned: a job's record is copied to
+   * the device on the stream while the host goes on) */
+  HBuf pin_jobrec;
+  JobRec* jobrec = nullptr;
+  /* pinned staging of each slot's pixel centres, and of the read-back of
+   * the jobs that complete after one launch */
+  HBuf pin_cen[kMaxJobs];
+  HBuf pin_readback;
   /* device memory of each job slot, kept from job to job (grown as needed,
    * given back when a job left it above kSlotKeepBytes): admitting a job
    * then allocates nothing */
@@ -2202,8 +2241,12 @@ void setup_job(monsel_ctx_s* C, Job& job) {
   C->slotbuf[job.slot].ensure(off);
   job.base = C->slotbuf[job.slot].as<char>();
   HIP_TRY(hipMemsetAsync(job.base, 0, job.zero_bytes, C->stream));
-  HIP_TRY(hipMemcpyAsync(job.base + job.o_cen, job.centers.data(), (size_t)np * 3 * 8, hipMemcpyHostToDevice,
-                         C->stream));
+  /* the slot's staging is free: its previous job's copies ended before that
+   * job completed */
+  HBuf& pc = C->pin_cen[job.slot];
+  pc.ensure((size_t)np * 3 * 8);
+  memcpy(pc.p, job.centers.data(), (size_t)np * 3 * 8);
+  HIP_TRY(hipMemcpyAsync(job.base + job.o_cen, pc.p, (size_t)np * 3 * 8, hipMemcpyHostToDevice, C->stream));
   const HostScene& H = job.sd->host;
   JobRec& R = C->jobrec[job.slot];
   memset(&R, 0, sizeof(R));
@@ -2243,17 +2286,20 @@ void setup_job(monsel_ctx_s* C, Job& job) {
     R.det_log = (monsel_detected*)(job.det.as<char>() + 16);
     R.det_max = (unsigned long long)jd.log_detected;
   }
-  /* the copies' sources (the job's centres, its record in C->jobrec) live
-   * until the job ends, and the next launch orders after them on the
-   * stream: no synchronisation here */
+  /* the copies' sources (pinned: the slot's centre staging, its record in
+   * C->jobrec) are not written again until the job ends, and the next
+   * launch orders after them on the stream: no synchronisation here */
   HIP_TRY(hipMemcpyAsync(C->jobtab.as<JobRec>() + job.slot, &R, sizeof(JobRec), hipMemcpyHostToDevice, C->stream));
 }
 
-/* Per-pixel reduction and read-back of a job whose electrons are all done. */
-void complete_job(monsel_ctx_s* C, Job& job) {
-  const monsel_job_desc& jd = job.desc;
-  const int np = jd.n_pixels;
-  const int nbins = C->jobrec[job.slot].nbins;
+/* bytes of a job's read-back: stats, histograms and per-pixel sums */
+size_t readback_bytes(const Job& job) { return job.o_px + (size_t)job.desc.n_pixels * 6 * 8; }
+
+/* Per-pixel reduction of a job whose electrons are all done, and the
+ * read-back of its tallies into `dst` (pinned, readback_bytes long), on the
+ * stream: the jobs that complete after one launch are read back together,
+ * with one synchronisation. */
+void complete_enqueue(monsel_ctx_s* C, Job& job, char* dst) {
   if (job.total > 0) {
     const int rb = 256;
     g_hip_call = "launch k_reduce_primaries";
@@ -2262,13 +2308,20 @@ void complete_job(monsel_ctx_s* C, Job& job) {
                        (unsigned long long*)(job.base + job.o_px));
     HIP_TRY(hipGetLastError());
   }
-  /* stats, histograms and per-pixel sums in one read-back */
-  std::vector<unsigned long long> all((job.o_px + (size_t)np * 6 * 8) / 8), st(8, 0ull);
-  HIP_TRY(hipMemcpyAsync(all.data(), job.base, all.size() * 8, hipMemcpyDeviceToHost, C->stream));
-  HIP_TRY(hipStreamSynchronize(C->stream));
-  const unsigned long long* sts = all.data();
-  const std::vector<unsigned long long> h(all.begin() + job.o_hist / 8, all.begin() + job.o_hist / 8 + hist_len(nbins));
-  const std::vector<unsigned long long> px(all.begin() + job.o_px / 8, all.begin() + job.o_px / 8 + (size_t)np * 6);
+  HIP_TRY(hipMemcpyAsync(dst, job.base, readback_bytes(job), hipMemcpyDeviceToHost, C->stream));
+}
+
+/* A completed job's results from its read-back (after the stream has
+ * passed complete_enqueue). */
+void complete_collect(monsel_ctx_s* C, Job& job, const char* src) {
+  const monsel_job_desc& jd = job.desc;
+  const int np = jd.n_pixels;
+  const int nbins = C->jobrec[job.slot].nbins;
+  const unsigned long long* all = (const unsigned long long*)src;
+  std::vector<unsigned long long> st(8, 0ull);
+  const unsigned long long* sts = all;
+  const std::vector<unsigned long long> h(all + job.o_hist / 8, all + job.o_hist / 8 + hist_len(nbins));
+  const std::vector<unsigned long long> px(all + job.o_px / 8, all + job.o_px / 8 + (size_t)np * 6);
   for (int k = 0; k < 8 * kStatStripes; ++k) st[k % 8] += sts[k];
   job.back.assign(px.begin(), px.begin() + np);
   job.fwd.assign(px.begin() + np, px.begin() + 2 * np);
@@ -2547,6 +2600,8 @@ void worker_main(monsel_ctx_s* C) {
         admitted.push_back(j);
       }
     }
+    /* jobs taken out of `active` to complete (finished ones are reset) */
+    std::vector<std::shared_ptr<Job>> ended;
     try {
       if (C->active.empty() && C->in_count == 0 && !C->batch_parts.empty()) {
         /* idle engine: start a fresh batch scene */
@@ -2559,6 +2614,9 @@ void worker_main(monsel_ctx_s* C) {
           setup_job(C, *j);
           C->active.push_back(j);
         } catch (const MonselError& e) {
+          /* copies it enqueued read the slot's pinned staging: they end
+           * before the slot is given to another job */
+          (void)hipStreamSynchronize(C->stream);
           if (j->slot >= 0) release_job(C, *j);
           finish_job(C, j, e.status, e.msg);
         }
@@ -2573,19 +2631,34 @@ void worker_main(monsel_ctx_s* C) {
           stop_job(C, *j);
         }
       run_generation(C);
-      /* completed jobs: all primaries launched and none of their electrons queued */
+      /* completed jobs: all primaries launched and none of their electrons
+       * queued; their reductions and read-backs go on the stream together */
       for (size_t k = 0; k < C->active.size();) {
-        auto j = C->active[k];
-        if (j->launched >= j->total && j->outstanding == 0) {
-          if (!j->cancelling) complete_job(C, *j);
-          release_job(C, *j);
+        if (C->active[k]->launched >= C->active[k]->total && C->active[k]->outstanding == 0) {
+          ended.push_back(C->active[k]);
           C->active.erase(C->active.begin() + k);
-          if (j->cancelling) finish_job(C, j, MONSEL_E_CANCELLED, "job cancelled");
-          else finish_job(C, j, MONSEL_OK, "");
         } else {
           ++k;
         }
       }
+      std::vector<size_t> at(ended.size() + 1, 0);
+      for (size_t k = 0; k < ended.size(); ++k)
+        at[k + 1] = at[k] + (ended[k]->cancelling ? 0 : (readback_bytes(*ended[k]) + 255) & ~(size_t)255);
+      if (at.back() > 0) {
+        C->pin_readback.ensure(at.back());
+        for (size_t k = 0; k < ended.size(); ++k)
+          if (!ended[k]->cancelling) complete_enqueue(C, *ended[k], C->pin_readback.as<char>() + at[k]);
+        HIP_TRY(hipStreamSynchronize(C->stream));
+      }
+      for (size_t k = 0; k < ended.size(); ++k) {
+        std::shared_ptr<Job> j = ended[k];
+        if (!j->cancelling) complete_collect(C, *j, C->pin_readback.as<char>() + at[k]);
+        release_job(C, *j);
+        ended[k].reset();
+        if (j->cancelling) finish_job(C, j, MONSEL_E_CANCELLED, "job cancelled");
+        else finish_job(C, j, MONSEL_OK, "");
+      }
+      ended.clear();
     } catch (const std::exception& ex) {
       /* a device failure ends every job in flight */
       std::string msg;
@@ -2597,6 +2670,8 @@ void worker_main(monsel_ctx_s* C) {
         msg = std::string(ex.what()) + " (in " + g_hip_call + ")";
       }
       (void)hipStreamSynchronize(C->stream);
+      for (auto& j : ended)
+        if (j) C->active.push_back(j);
       for (auto& j : C->active) {
         release_job(C, *j);
         finish_job(C, j, st, msg);
@@ -2653,8 +2728,12 @@ int monsel_create(int device_ordinal, const monsel_tuning* tuning, monsel_ctx* o
     C->hbm = prop.totalGlobalMem;
     C->name = prop.gcnArchName;
     HIP_TRY(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
-    HIP_TRY(hipHostMalloc((void**)&C->host_count, (4 + kMaxJobs) * sizeof(unsigned int), hipHostMallocDefault));
-    HIP_TRY(hipHostMalloc((void**)&C->host_segs, 3 * kMaxJobs * sizeof(uint32_t), hipHostMallocDefault));
+    C->pin_count.ensure((4 + kMaxJobs) * sizeof(unsigned int));
+    C->host_count = C->pin_count.as<unsigned int>();
+    C->pin_segs.ensure(3 * kMaxJobs * sizeof(uint32_t));
+    C->host_segs = C->pin_segs.as<uint32_t>();
+    C->pin_jobrec.ensure(kMaxJobs * sizeof(JobRec));
+    C->jobrec = C->pin_jobrec.as<JobRec>();
     if (C->tuning.max_jobs_in_flight < 0 || C->tuning.primaries_per_launch < 0 || C->tuning.steps_per_launch < 0 ||
         C->tuning.queue_capacity < 0 || C->tuning.admission_order < 0 || C->tuning.admission_order > 1 ||
         C->tuning.sort_queue < -1 || C->tuning.sort_queue > 0 || C->tuning.geometry_batch < 0 ||
@@ -2708,8 +2787,6 @@ int monsel_destroy(monsel_ctx ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   ctx->scenes.clear();
   ctx->jobs.clear();
-  if (ctx->host_count) (void)hipHostFree(ctx->host_count);
-  if (ctx->host_segs) (void)hipHostFree(ctx->host_segs);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->evk0) (void)hipEventDestroy(ctx->evk0);

@@ -292,3 +292,51 @@ def serve(engine, requests, plain=False, hbm_budget=64 << 30, time_scale=1.0, po
     if keep_results:
         out["results"] = kept
     return out
+
+
+def batch_timing(engine, jobs):
+    """run_batch's batched form with the host side of its wall time split
+    out: the submit loop, the last job's completion on the engine (its
+    submit time plus its submit-to-done time), and the reading of the
+    results after it."""
+    cache = SceneCache(engine)
+    for j in jobs:
+        _yield_scene(cache, j)
+    t0 = time.perf_counter()
+    ids, t_sub = [], []
+    for j in jobs:
+        sc, cs, sid = _yield_scene(cache, j)
+        t_sub.append(time.perf_counter())
+        ids.append(engine.submit(sid, [[0.0, 0.0, sc.beam_z]], j.primaries, j.beam_eV, 0.5e-9, j.seed))
+    t1 = time.perf_counter()
+    results = [engine.result(i) for i in ids]
+    wall = time.perf_counter() - t0
+    last = max(ts + r["stats"]["total_ms"] * 1e-3 for ts, r in zip(t_sub, results)) - t0
+    prim = sum(j.primaries for j in jobs)
+    return results, dict(jobs=len(jobs), primaries=prim, wall_s=wall, submit_s=t1 - t0, last_done_s=last,
+                         read_after_last_s=wall - last, primaries_per_s=prim / wall)
+
+
+def main(argv=None):
+    """python -m cudamonsel_amd.serving batch [--jobs N] [--tuning FILE]:
+    the batch workload (N single-pixel yield jobs, seed 4) on a tuned engine,
+    after a warm-up of its first 8 jobs; prints one JSON line."""
+    import argparse
+    import json
+    import os
+    from .abi import Engine
+    ap = argparse.ArgumentParser(prog="python -m cudamonsel_amd.serving")
+    ap.add_argument("form", choices=["batch"])
+    ap.add_argument("--jobs", type=int, default=512)
+    ap.add_argument("--tuning", default=os.path.join(os.path.dirname(__file__), "data", "tuning_batch.json"))
+    a = ap.parse_args(argv)
+    tuning = json.load(open(a.tuning))["tuning"] if os.path.exists(a.tuning) else {}
+    jobs = batch_workload(a.jobs, seed=4)
+    with Engine(**tuning) as eng:
+        run_batch(eng, jobs[:8])
+        _, t = batch_timing(eng, jobs)
+    print(json.dumps(dict(t, tuning=tuning)), flush=True)
+
+
+if __name__ == "__main__":
+    main()
