@@ -1085,78 +1085,92 @@ __device__ __forceinline__ uint32_t sort_key(const uint32_t* in_u, const float* 
   return k < (uint32_t)n_keys ? k : (uint32_t)n_keys - 1u;
 }
 
-__global__ void __launch_bounds__(256) k_sort_count(const float* in_f, const uint32_t* in_u, uint32_t cap,
-                                                    const unsigned int* count, unsigned int* hist, int n_keys) {
+/* The sort runs in three passes over chunks of the queue, one chunk per
+ * block: k_sort_hist writes each chunk's count per key into a key-major
+ * table (table[key * n_blocks + block]), k_sort_scan turns the table into
+ * exclusive offsets (so that the entries of one key are grouped, chunk by
+ * chunk), and k_sort_place writes each chunk's entries of a key to the
+ * contiguous range its table entry names.  No global atomics: a chunk's
+ * entries of one key land in one run of slots, written by one block. */
+constexpr uint32_t kSortChunk = 1024;           /* entries per block at least */
+constexpr uint32_t kSortTable = 256u * 1024u;   /* table entries: n_keys x blocks at most */
+
+/* one LDS atomic per distinct key of the wave; returns the lane's rank among
+ * the wave's lanes of its key plus the key's base from the LDS counter */
+__device__ __forceinline__ uint32_t wave_key_slot(unsigned int* lc, bool valid, uint32_t key) {
+  const unsigned long long lanemask_lt = (1ull << lane_id()) - 1ull;
+  uint32_t slot = 0;
+  unsigned long long left = wave_ballot(valid);
+  while (left) {
+    const uint32_t k0 = __shfl(key, __ffsll((long long)left) - 1);
+    const unsigned long long same = wave_ballot(valid && key == k0);
+    const int leader = __ffsll((long long)same) - 1;
+    uint32_t b = 0;
+    if (lane_id() == leader) b = atomicAdd(&lc[k0], (unsigned int)__popcll(same));
+    b = __shfl(b, leader);
+    if (valid && key == k0) slot = b + (uint32_t)__popcll(same & lanemask_lt);
+    left &= ~same;
+  }
+  return slot;
+}
+
+__global__ void __launch_bounds__(256) k_sort_hist(const float* __restrict__ in_f, const uint32_t* __restrict__ in_u,
+                                                   uint32_t cap, uint32_t n, uint32_t chunk, int n_keys,
+                                                   unsigned int* __restrict__ table) {
   __shared__ unsigned int lc[kSortKeys];
   for (int i = threadIdx.x; i < n_keys; i += blockDim.x) lc[i] = 0u;
   __syncthreads();
-  const uint32_t n = min(*count, cap);
-  const uint32_t stride = gridDim.x * blockDim.x;
-  /* uniform trip count per wave so that the ballots below see every lane */
-  for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
+  const uint32_t first = blockIdx.x * chunk;
+  const uint32_t last = min(n, first + chunk);
+  /* uniform trip count per wave so that the ballots see every lane */
+  for (uint32_t base = first + (threadIdx.x & ~63u); base < last; base += blockDim.x) {
     const uint32_t i = base + lane_id();
-    const bool valid = i < n;
-    const uint32_t key = valid ? sort_key(in_u, in_f, cap, i, n_keys) : 0xFFFFFFFFu;
-    /* one LDS atomic per distinct key of the wave */
-    unsigned long long left = wave_ballot(valid);
-    while (left) {
-      const uint32_t k0 = __shfl(key, __ffsll((long long)left) - 1);
-      const unsigned long long same = wave_ballot(key == k0);
-      if (lane_id() == __ffsll((long long)same) - 1) atomicAdd(&lc[k0], (unsigned int)__popcll(same));
-      left &= ~same;
-    }
+    const bool valid = i < last;
+    const uint32_t key = valid ? sort_key(in_u, in_f, cap, i, n_keys) : 0u;
+    (void)wave_key_slot(lc, valid, key);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < n_keys; i += blockDim.x)
-    if (lc[i]) atomicAdd(&hist[i], lc[i]);
+  for (int k = threadIdx.x; k < n_keys; k += blockDim.x) table[(size_t)k * gridDim.x + blockIdx.x] = lc[k];
 }
 
-/* exclusive prefix sum of hist[0..n_keys) into offs (one block) */
-__global__ void __launch_bounds__(1024) k_sort_offsets(const unsigned int* hist, unsigned int* offs, int n_keys) {
+/* exclusive prefix sum of table[0..total) in place (one block) */
+__global__ void __launch_bounds__(1024) k_sort_scan(unsigned int* table, uint32_t total) {
   __shared__ unsigned int part[1024];
-  const int per = (n_keys + 1023) / 1024;
-  const int t = threadIdx.x;
+  const uint32_t per = (total + 1023u) / 1024u;
+  const uint32_t t = threadIdx.x;
+  const uint32_t lo = min(total, t * per), hi = min(total, lo + per);
   unsigned int s = 0;
-  for (int k = t * per; k < min(n_keys, (t + 1) * per); ++k) s += hist[k];
+  for (uint32_t k = lo; k < hi; ++k) s += table[k];
   part[t] = s;
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
+  for (uint32_t off = 1; off < 1024u; off <<= 1) {
     const unsigned int v = t >= off ? part[t - off] : 0u;
     __syncthreads();
     part[t] += v;
     __syncthreads();
   }
   unsigned int run = t > 0 ? part[t - 1] : 0u;
-  for (int k = t * per; k < min(n_keys, (t + 1) * per); ++k) {
-    offs[k] = run;
-    run += hist[k];
+  for (uint32_t k = lo; k < hi; ++k) {
+    const unsigned int c = table[k];
+    table[k] = run;
+    run += c;
   }
 }
 
-__global__ void __launch_bounds__(256) k_sort_scatter(const float* in_f, const uint32_t* in_u, float* out_f,
-                                                      uint32_t* out_u, uint32_t cap, const unsigned int* count,
-                                                      unsigned int* offs, int n_keys) {
-  const uint32_t n = min(*count, cap);
-  const uint32_t stride = gridDim.x * blockDim.x;
-  const unsigned long long lanemask_lt = (1ull << lane_id()) - 1ull;
-  for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
+__global__ void __launch_bounds__(256) k_sort_place(const float* __restrict__ in_f, const uint32_t* __restrict__ in_u,
+                                                    float* __restrict__ out_f, uint32_t* __restrict__ out_u,
+                                                    uint32_t cap, uint32_t n, uint32_t chunk, int n_keys,
+                                                    const unsigned int* __restrict__ table) {
+  __shared__ unsigned int lc[kSortKeys];
+  for (int k = threadIdx.x; k < n_keys; k += blockDim.x) lc[k] = table[(size_t)k * gridDim.x + blockIdx.x];
+  __syncthreads();
+  const uint32_t first = blockIdx.x * chunk;
+  const uint32_t last = min(n, first + chunk);
+  for (uint32_t base = first + (threadIdx.x & ~63u); base < last; base += blockDim.x) {
     const uint32_t i = base + lane_id();
-    const bool valid = i < n;
-    const uint32_t key = valid ? sort_key(in_u, in_f, cap, i, n_keys) : 0xFFFFFFFFu;
-    /* one global atomic per distinct key of the wave; lanes of a key take
-     * consecutive slots in lane order */
-    uint32_t slot = 0;
-    unsigned long long left = wave_ballot(valid);
-    while (left) {
-      const uint32_t k0 = __shfl(key, __ffsll((long long)left) - 1);
-      const unsigned long long same = wave_ballot(key == k0);
-      const int leader = __ffsll((long long)same) - 1;
-      uint32_t b = 0;
-      if (lane_id() == leader) b = atomicAdd(&offs[k0], (unsigned int)__popcll(same));
-      b = __shfl(b, leader);
-      if (key == k0) slot = b + (uint32_t)__popcll(same & lanemask_lt);
-      left &= ~same;
-    }
+    const bool valid = i < last;
+    const uint32_t key = valid ? sort_key(in_u, in_f, cap, i, n_keys) : 0u;
+    const uint32_t slot = wave_key_slot(lc, valid, key);
     if (valid) {
 #pragma unroll
       for (int f = 0; f < kQF; ++f) out_f[(size_t)f * cap + slot] = in_f[(size_t)f * cap + i];
@@ -2107,7 +2121,7 @@ struct monsel_ctx_s {
   bool stop = false;
   std::thread worker;
   /* engine (worker thread only) */
-  DBuf q[2], esc, counts, segs, jobtab;
+  DBuf q[2], esc, counts, segs, jobtab, sort_table;
   uint32_t qcap = 0;          /* entries per queue buffer */
   int cur = 0;                /* queue holding the next launch's input */
   uint32_t in_count = 0;      /* electrons in q[cur] */
@@ -2445,7 +2459,7 @@ void run_generation(monsel_ctx_s* C) {
   }
   unsigned int* counts = C->counts.as<unsigned int>();
   /* counts[0] claim, [1] queued electrons, [2] escapes, [4..] per-job queue counts */
-  HIP_TRY(hipMemsetAsync(counts, 0, (4 + kMaxJobs) * sizeof(unsigned int), C->stream));  /* sort scratch follows */
+  HIP_TRY(hipMemsetAsync(counts, 0, (4 + kMaxJobs) * sizeof(unsigned int), C->stream));
   if (nseg > 0)
     HIP_TRY(hipMemcpyAsync(C->segs.p, C->host_segs, 3 * kMaxJobs * 4, hipMemcpyHostToDevice, C->stream));
   const int cur = C->cur;
@@ -2530,23 +2544,30 @@ void run_generation(monsel_ctx_s* C) {
   /* regroup the queue by (region, energy class) for the next launch */
   const int n_keys = 2 * C->batch.dev.n_regions;
   if (C->tuning.sort_queue >= 0 && C->in_count >= 65536 && n_keys <= kSortKeys) {
-    unsigned int* hist = counts + 4 + kMaxJobs;
-    unsigned int* offs = hist + kSortKeys;
+    const uint32_t n = C->in_count;
+    /* chunks of kSortChunk entries, larger when the key x chunk table would
+     * exceed kSortTable entries (then at least kSortTable / n_keys chunks) */
+    uint32_t chunk = kSortChunk;
+    uint64_t nb = ((uint64_t)n + chunk - 1) / chunk;
+    if (nb * (uint64_t)n_keys > kSortTable) {
+      const uint64_t maxb = kSortTable / (uint64_t)n_keys;
+      chunk = (uint32_t)((((uint64_t)n + maxb - 1) / maxb + 255) / 256 * 256);
+      nb = ((uint64_t)n + chunk - 1) / chunk;
+    }
+    unsigned int* table = C->sort_table.as<unsigned int>();
     const float* sf = C->q[C->cur].as<float>();
     const uint32_t* su = (const uint32_t*)(sf + (size_t)kQF * cap);
     float* df = C->q[C->cur ^ 1].as<float>();
     uint32_t* du = (uint32_t*)(df + (size_t)kQF * cap);
-    HIP_TRY(hipMemsetAsync(hist, 0, (size_t)n_keys * sizeof(unsigned int), C->stream));
-    g_hip_call = "launch k_sort_count";
-    hipLaunchKernelGGL(k_sort_count, dim3(C->n_cu * 2), dim3(256), 0, C->stream, sf, su, cap,
-                       (const unsigned int*)(counts + 1), hist, n_keys);
+    g_hip_call = "launch k_sort_hist";
+    hipLaunchKernelGGL(k_sort_hist, dim3((unsigned)nb), dim3(256), 0, C->stream, sf, su, cap, n, chunk, n_keys, table);
     HIP_TRY(hipGetLastError());
-    g_hip_call = "launch k_sort_offsets";
-    hipLaunchKernelGGL(k_sort_offsets, dim3(1), dim3(1024), 0, C->stream, (const unsigned int*)hist, offs, n_keys);
+    g_hip_call = "launch k_sort_scan";
+    hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, C->stream, table, (uint32_t)(nb * (uint64_t)n_keys));
     HIP_TRY(hipGetLastError());
-    g_hip_call = "launch k_sort_scatter";
-    hipLaunchKernelGGL(k_sort_scatter, dim3(C->n_cu * 8), dim3(256), 0, C->stream, sf, su, df, du, cap,
-                       (const unsigned int*)(counts + 1), offs, n_keys);
+    g_hip_call = "launch k_sort_place";
+    hipLaunchKernelGGL(k_sort_place, dim3((unsigned)nb), dim3(256), 0, C->stream, sf, su, df, du, cap, n, chunk,
+                       n_keys, (const unsigned int*)table);
     HIP_TRY(hipGetLastError());
     C->cur ^= 1;
   }
@@ -2741,7 +2762,8 @@ int monsel_create(int device_ordinal, const monsel_tuning* tuning, monsel_ctx* o
         C->tuning.tail_requeue < -1 || C->tuning.tail_requeue > 64)
       throw MonselError(MONSEL_E_INVALID, "tuning value out of range");
     C->jobtab.ensure(kMaxJobs * sizeof(JobRec));
-    C->counts.ensure((4 + kMaxJobs + 2 * kSortKeys) * sizeof(unsigned int));
+    C->counts.ensure((4 + kMaxJobs) * sizeof(unsigned int));
+    C->sort_table.ensure(kSortTable * sizeof(unsigned int));
     C->segs.ensure(3 * kMaxJobs * sizeof(uint32_t));
     for (int k = kMaxJobs - 1; k >= 0; --k) C->free_slots.push_back(k);
     C->qcap = queue_capacity(C.get());

@@ -74,7 +74,10 @@ def test_tail_compaction_keeps_results(M):
     to a few live lanes queue them for the next launch; the results do not
     depend on the threshold.  Regrouping every 7 steps with secondaries handed
     to free lanes, twice: whether an electron is queued again (which runs the
-    containment test at its claim) or keeps its lane does not change it."""
+    containment test at its claim) or keeps its lane does not change it.
+    The requeued queue (over 65536 electrons) is sorted by region and energy
+    class between launches; the same run with the queue left unsorted
+    matches, so the sort neither drops nor repeats an electron."""
     from cudamonsel_amd import scenes
     from cudamonsel_amd.abi import Engine
     sc = scenes.line_on_substrate()
@@ -82,7 +85,8 @@ def test_tail_compaction_keeps_results(M):
     cen = scenes.scan_centers(np.linspace(-20, 20, 64), [0.0], sc.beam_z)
     res = []
     for tuning in (dict(tail_requeue=-1), dict(), dict(tail_requeue=48), dict(tail_requeue=48, geometry_batch=8),
-                   dict(steps_per_launch=7, spawn_handoff=-1), dict(steps_per_launch=7, spawn_handoff=-1)):
+                   dict(steps_per_launch=7, spawn_handoff=-1), dict(steps_per_launch=7, spawn_handoff=-1),
+                   dict(steps_per_launch=7, sort_queue=-1)):
         with Engine(**tuning) as eng:
             r = eng.result(eng.submit(eng.upload(cs), cen, 1100, 500.0, 0.5e-9, 21))
             assert r["stats"]["queue_overflow"] == 0 and r["stats"]["watchdog_drops"] == 0
