@@ -13,6 +13,9 @@ winner can be stored as JSON and passed to `abi.Engine(**tuning)`.
 
 Usage: python -m cudamonsel_amd.autotune batch [out.json]
        (the 512-job batch workload of serving.batch_workload)
+       python -m cudamonsel_amd.autotune linescan [out.json]
+       (line-scan jobs of 128 pixels x 2000 primaries across a PMMA line on
+       Si at 0.5 keV, submitted back to back with 4 in flight)
 """
 import json
 import os
@@ -93,6 +96,53 @@ def load(path):
         return json.load(f)["tuning"]
 
 
+# the line scan starts from its measured tuning; the regroup interval and
+# the tail threshold get finer steps (their best values moved with the cost
+# of the between-launch queue sort)
+LINESCAN_SPACE = {
+    "primaries_per_launch": [262144, 131072, 524288, 0],
+    "steps_per_launch": [0, 512, 256, 128, 64],
+    "tail_requeue": [48, 32, 64, 16, -1],
+    "geometry_batch": [8, 4, 12, 16],
+    "sort_queue": [0, -1],
+    "spawn_handoff": [0, -1],
+    "admission_order": [0, 1],
+}
+
+
+def linescan_workload(jobs=6, inflight=4, per_pixel=2000):
+    """workload(engine) for the line scan: the scene is uploaded once per
+    engine, then `jobs` scan jobs run with up to `inflight` on the device."""
+    import numpy as np
+    from . import scenes
+    sc = scenes.line_on_substrate(h_nm=50.0, w_nm=20.0, theta_deg=3.0, r_nm=5.0, length_nm=1000.0, line="PMMA",
+                                  substrate="Si", beam_eV=500.0, sigma_nm=0.5, compat=True)
+    centers = scenes.scan_centers(np.arange(-64.0, 64.0, 1.0), [0.0], sc.beam_z)
+    cs = sc.monte.compileScene()
+
+    def workload(eng):
+        sid = getattr(eng, "_linescan_scene", None)
+        if sid is None:
+            sid = eng._linescan_scene = eng.upload(cs)
+            eng.result(eng.submit(sid, centers, per_pixel, 500.0, 0.5e-9, 999))  # warm-up
+        ids = []
+        for k in range(jobs):
+            ids.append(eng.submit(sid, centers, per_pixel, 500.0, 0.5e-9, 1000 + k))
+            if len(ids) >= inflight:
+                eng.result(ids.pop(0))
+        while ids:
+            eng.result(ids.pop(0))
+    return workload
+
+
+def _linescan_main(out):
+    import sys
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    best, trials = autotune(linescan_workload(), space=LINESCAN_SPACE, log=log)
+    save(out, best, trials)
+    print(json.dumps(best))
+
+
 def _batch_main(out):
     import sys
     from . import serving
@@ -108,6 +158,11 @@ def _batch_main(out):
 
 if __name__ == "__main__":
     import sys
+    here = os.path.dirname(os.path.abspath(__file__))
     if len(sys.argv) > 1 and sys.argv[1] == "batch":
-        here = os.path.dirname(os.path.abspath(__file__))
         _batch_main(sys.argv[2] if len(sys.argv) > 2 else os.path.join(here, "data", "tuning_batch.json"))
+    elif len(sys.argv) > 1 and sys.argv[1] == "linescan":
+        _linescan_main(sys.argv[2] if len(sys.argv) > 2 else os.path.join(here, "data", "tuning_linescan.json"))
+    else:
+        print(__doc__, file=sys.stderr)
+        sys.exit(2)

@@ -33,3 +33,37 @@ def test_noise_margin(monkeypatch):
     monkeypatch.setattr(autotune, "_time", lambda w, t, d, repeats=1: 1.0 if t["a"] == 0 else 0.995)
     best, _ = autotune.autotune(None, space=space, min_gain=0.01)
     assert best == {"a": 0}
+
+
+def test_linescan_workload_uploads_once_per_engine():
+    """The line-scan workload uploads its scene and runs its warm-up job once
+    per engine, then submits its jobs with at most `inflight` unfinished."""
+    from cudamonsel_amd import autotune
+
+    class FakeEngine:
+        def __init__(self):
+            self.uploads, self.open, self.peak, self.submitted, self.next = 0, set(), 0, [], 1
+
+        def upload(self, cs):
+            self.uploads += 1
+            return 7
+
+        def submit(self, sid, centers, per, e, sigma, seed):
+            assert sid == 7 and centers.shape == (128, 3) and per == 2000 and e == 500.0
+            self.next += 1
+            self.open.add(self.next)
+            self.peak = max(self.peak, len(self.open))
+            self.submitted.append(seed)
+            return self.next
+
+        def result(self, jid):
+            self.open.remove(jid)
+            return {}
+
+    w = autotune.linescan_workload(jobs=6, inflight=4)
+    a, b = FakeEngine(), FakeEngine()
+    w(a)
+    w(a)
+    w(b)
+    assert a.uploads == 1 and b.uploads == 1
+    assert a.submitted == [999] + list(range(1000, 1006)) * 2 and a.peak == 4 and not a.open
