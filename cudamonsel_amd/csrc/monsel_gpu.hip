@@ -1133,28 +1133,64 @@ __global__ void __launch_bounds__(256) k_sort_hist(const float* __restrict__ in_
   for (int k = threadIdx.x; k < n_keys; k += blockDim.x) table[(size_t)k * gridDim.x + blockIdx.x] = lc[k];
 }
 
-/* exclusive prefix sum of table[0..total) in place (one block) */
-__global__ void __launch_bounds__(1024) k_sort_scan(unsigned int* table, uint32_t total) {
-  __shared__ unsigned int part[1024];
-  const uint32_t per = (total + 1023u) / 1024u;
-  const uint32_t t = threadIdx.x;
-  const uint32_t lo = min(total, t * per), hi = min(total, lo + per);
-  unsigned int s = 0;
-  for (uint32_t k = lo; k < hi; ++k) s += table[k];
-  part[t] = s;
+/* Exclusive prefix sum of table[0..total) in place, in tiles of
+ * kScanTile entries (256 threads x 4): k_sort_tile_sums sums each tile,
+ * k_sort_scan_tiles scans the tile sums (at most kSortTable / kScanTile =
+ * 256 of them) in every block and then its own tile. */
+constexpr uint32_t kScanTile = 1024;
+
+/* inclusive scan of one value per thread of a 256-thread block */
+__device__ __forceinline__ unsigned int block_scan_256(unsigned int v, unsigned int* l) {
+  const int t = threadIdx.x;
+  l[t] = v;
   __syncthreads();
-  for (uint32_t off = 1; off < 1024u; off <<= 1) {
-    const unsigned int v = t >= off ? part[t - off] : 0u;
+  for (int off = 1; off < 256; off <<= 1) {
+    const unsigned int a = t >= off ? l[t - off] : 0u;
     __syncthreads();
-    part[t] += v;
+    l[t] += a;
     __syncthreads();
   }
-  unsigned int run = t > 0 ? part[t - 1] : 0u;
-  for (uint32_t k = lo; k < hi; ++k) {
-    const unsigned int c = table[k];
-    table[k] = run;
-    run += c;
+  const unsigned int r = l[t];
+  __syncthreads();
+  return r;
+}
+
+__global__ void __launch_bounds__(256) k_sort_tile_sums(const unsigned int* __restrict__ table, uint32_t total,
+                                                        unsigned int* __restrict__ sums) {
+  __shared__ unsigned int l[256];
+  const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * 4u;
+  unsigned int s = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 4u; ++k)
+    if (base + k < total) s += table[base + k];
+  s = block_scan_256(s, l);
+  if (threadIdx.x == 255) sums[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(256) k_sort_scan_tiles(unsigned int* __restrict__ table, uint32_t total,
+                                                         const unsigned int* __restrict__ sums) {
+  __shared__ unsigned int l[256];
+  /* this tile's offset: the sum of the tile sums before it */
+  const unsigned int mine = threadIdx.x < blockIdx.x ? sums[threadIdx.x] : 0u;
+  const unsigned int before = block_scan_256(mine, l);
+  __shared__ unsigned int tile_off;
+  if (threadIdx.x == 255) tile_off = before;
+  __syncthreads();
+  const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * 4u;
+  unsigned int v[4];
+  unsigned int s = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 4u; ++k) {
+    v[k] = base + k < total ? table[base + k] : 0u;
+    s += v[k];
   }
+  unsigned int run = tile_off + block_scan_256(s, l) - s;
+#pragma unroll
+  for (uint32_t k = 0; k < 4u; ++k)
+    if (base + k < total) {
+      table[base + k] = run;
+      run += v[k];
+    }
 }
 
 __global__ void __launch_bounds__(256) k_sort_place(const float* __restrict__ in_f, const uint32_t* __restrict__ in_u,
@@ -2562,8 +2598,15 @@ void run_generation(monsel_ctx_s* C) {
     g_hip_call = "launch k_sort_hist";
     hipLaunchKernelGGL(k_sort_hist, dim3((unsigned)nb), dim3(256), 0, C->stream, sf, su, cap, n, chunk, n_keys, table);
     HIP_TRY(hipGetLastError());
-    g_hip_call = "launch k_sort_scan";
-    hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, C->stream, table, (uint32_t)(nb * (uint64_t)n_keys));
+    const uint32_t total = (uint32_t)(nb * (uint64_t)n_keys);
+    const uint32_t tiles = (total + kScanTile - 1) / kScanTile;   /* <= 256 */
+    unsigned int* sums = table + kSortTable;
+    g_hip_call = "launch k_sort_tile_sums";
+    hipLaunchKernelGGL(k_sort_tile_sums, dim3(tiles), dim3(256), 0, C->stream, (const unsigned int*)table, total, sums);
+    HIP_TRY(hipGetLastError());
+    g_hip_call = "launch k_sort_scan_tiles";
+    hipLaunchKernelGGL(k_sort_scan_tiles, dim3(tiles), dim3(256), 0, C->stream, table, total,
+                       (const unsigned int*)sums);
     HIP_TRY(hipGetLastError());
     g_hip_call = "launch k_sort_place";
     hipLaunchKernelGGL(k_sort_place, dim3((unsigned)nb), dim3(256), 0, C->stream, sf, su, df, du, cap, n, chunk,
@@ -2763,7 +2806,7 @@ int monsel_create(int device_ordinal, const monsel_tuning* tuning, monsel_ctx* o
       throw MonselError(MONSEL_E_INVALID, "tuning value out of range");
     C->jobtab.ensure(kMaxJobs * sizeof(JobRec));
     C->counts.ensure((4 + kMaxJobs) * sizeof(unsigned int));
-    C->sort_table.ensure(kSortTable * sizeof(unsigned int));
+    C->sort_table.ensure((kSortTable + kSortTable / kScanTile) * sizeof(unsigned int));
     C->segs.ensure(3 * kMaxJobs * sizeof(uint32_t));
     for (int k = kMaxJobs - 1; k >= 0; --k) C->free_slots.push_back(k);
     C->qcap = queue_capacity(C.get());

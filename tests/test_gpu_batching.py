@@ -199,3 +199,24 @@ def test_cancel(M):
         assert eng.wait_status(done, timeout_s=60) == 0
         eng.cancel(done)
         _same(alone, eng.result(done))
+
+
+def test_queue_sort_with_many_regions(M):
+    """The between-launch sort on a scene of 148 regions (12 x 12 contact
+    holes: 296 sort keys, a key x chunk table of many scan tiles), with
+    every electron requeued after 7 steps so that each launch's queue is
+    sorted: results equal the run with the queue left unsorted."""
+    from cudamonsel_amd import scenes
+    from cudamonsel_amd.abi import Engine
+    sc = scenes.contact_holes(radius_nm=6.0, depth_nm=40.0, pitch_nm=16.0, n=12)
+    cs = sc.monte.compileScene()
+    xs = np.linspace(-80.0, 80.0, 8)
+    cen = scenes.scan_centers(xs, xs, sc.beam_z)
+    res = []
+    for tuning in (dict(steps_per_launch=7), dict(steps_per_launch=7, sort_queue=-1)):
+        with Engine(**tuning) as eng:
+            r = eng.result(eng.submit(eng.upload(cs), cen, 1200, 500.0, 0.5e-9, 31))
+            assert r["stats"]["queue_overflow"] == 0 and r["stats"]["watchdog_drops"] == 0
+            res.append(r)
+    assert res[0]["stats"]["transport_launches"] > 3
+    _same(res[0], res[1])
