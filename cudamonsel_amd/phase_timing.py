@@ -67,8 +67,12 @@ def launches(spans):
 
 
 def table(c):
-    """Rows (phase, share of loop cycles, cycles per entry, entries, lane utilisation)."""
-    loop = c[PHASES.index("claim"), 0] + c[PHASES.index("step"), 0] + c[PHASES.index("tail"), 0]
+    """Rows (phase, share of loop cycles, cycles per entry, entries, lane utilisation).
+    The loop's cycles are those of its four top-level phases: the claim, the
+    step's start (rates), the step's end (step) and the tail; the other
+    phases lie within one of them."""
+    loop = (c[PHASES.index("claim"), 0] + c[PHASES.index("rates"), 0] + c[PHASES.index("step"), 0] +
+            c[PHASES.index("tail"), 0])
     rows = []
     for k, name in enumerate(PHASES):
         cyc, ent, lanes = c[k]

@@ -297,3 +297,22 @@ def line_scan_spec(h_nm=50.0, w_nm=20.0, theta_deg=3.0, r_nm=5.0, length_nm=1000
         "beam": {"energy_eV": beam_eV, "sigma_nm": sigma_nm, "center_nm": [0.0, 0.0, -(h_nm + 20.0)]},
         "ref_compat": "reference" if compat else "fixed",
     }
+
+
+def main(argv=None):
+    """python -m cudamonsel_amd.scenefile JOB.json [JOB.json ...]: runs each
+    job file on GPU 0 and writes the outputs it names."""
+    import sys
+    paths = sys.argv[1:] if argv is None else argv
+    if not paths:
+        print(main.__doc__, file=sys.stderr)
+        return 2
+    for p in paths:
+        r, cen = run_job(p)
+        print(f"{p}: {cen.shape[0]} pixels, {r['stats']['primaries']} primaries, "
+              f"{r['stats']['steps']} electron steps", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
