@@ -38,6 +38,9 @@ enum Phase {
   PH_SAFETY,       /* the safety bound at a step's end point */
   PH_UNIFORMS,     /* the step's first four uniforms (within PH_RATES) */
   PH_RATE_LOOP,    /* the mechanism rates (within PH_RATES) */
+  PH_GEOM_IV,      /* primitive entry/exit intervals (within PH_GEOM) */
+  PH_GEOM_WALK,    /* crossing walks of the shapes (within PH_GEOM) */
+  PH_GEOM_LOC,     /* locating the region beyond a crossing (within PH_GEOM) */
   kPhases
 };
 

@@ -1162,10 +1162,13 @@ MD ShapeHit shape_first(const DevScene& S, const MONSEL_CS DShape& sh, float px,
   };
   /* the first four primitives unrolled, so that their records' loads and
    * interval arithmetic can overlap */
+  PT_BEGIN(t_iv);
 #pragma unroll
   for (int k = 0; k < 4; ++k)
     if (k < n) one(k);
   for (int k = 4; k < n; ++k) one(k);
+  PT_END(PH_GEOM_IV, t_iv);
+  PT_BEGIN(t_wk);
   bool state = truth(sh, bits);
   float tcur = 0.f;
   for (int it = 0; it < 2 * n; ++it) {
@@ -1184,7 +1187,10 @@ MD ShapeHit shape_first(const DevScene& S, const MONSEL_CS DShape& sh, float px,
         mask |= 1u << k;
       }
     }
-    if (!(tnext <= tmax)) return h;
+    if (!(tnext <= tmax)) {
+      PT_END(PH_GEOM_WALK, t_wk);
+      return h;
+    }
     bits ^= mask;
     const bool ns = truth(sh, bits);
     if (ns != state) {
@@ -1209,11 +1215,13 @@ MD ShapeHit shape_first(const DevScene& S, const MONSEL_CS DShape& sh, float px,
       h.nx = nx;
       h.ny = ny;
       h.nz = nz;
+      PT_END(PH_GEOM_WALK, t_wk);
       return h;
     }
     state = ns;
     tcur = tnext;
   }
+  PT_END(PH_GEOM_WALK, t_wk);
   return h;
 }
 
@@ -1390,11 +1398,13 @@ MD StepEnd find_end_of_step(const DevScene& S, int r, float px, float py, float 
     float dd = nudge_nm(qx, qy, qz);
     float ox = qx + dd * dx, oy = qy + dd * dy, oz = qz + dd * dz;
     int res = -1;
+    PT_BEGIN(t_lo);
     for (int guard = 0; guard < 64 && base >= 0; ++guard) {
       res = containing_sub_region(S, base, ox, oy, oz);
       if (res >= 0) break;
       base = S.regions[base].parent;
     }
+    PT_END(PH_GEOM_LOC, t_lo);
     se.next = res;
     return se;
   }

@@ -16,7 +16,8 @@ import sys
 import numpy as np
 
 PHASES = ["claim", "contain", "rates", "geom", "csd", "select", "elastic", "fitted", "polaron", "phonon",
-          "tabulated", "barrier", "step", "tail", "rates_uni", "geom_uni", "safety", "uniforms", "rate_loop"]
+          "tabulated", "barrier", "step", "tail", "rates_uni", "geom_uni", "safety", "uniforms", "rate_loop",
+          "geom_iv", "geom_walk", "geom_loc"]
 
 
 def read_counters(lib, device=0):
