@@ -174,8 +174,10 @@ def batched_leg(n_jobs, tuning):
         res_p, t_p = serving.run_batch(eng, jobs, plain=True)
     log(f"plain: {t_p:.3f} s")
     same = all(np.array_equal(a["back"], b["back"]) and np.array_equal(a["se"], b["se"]) for a, b in zip(res, res_p))
+    steps = sum(int(r["stats"]["steps"]) for r in res)
     return {"jobs": n_jobs, "primaries": prim, "batched_primaries_per_s": round(prim / t_b, 1),
             "plain_primaries_per_s": round(prim / t_p, 1), "speedup": round(t_p / t_b, 2),
+            "electron_steps": steps, "batched_electron_steps_per_s": round(steps / t_b, 1),
             "batched_s": round(t_b, 3), "plain_s": round(t_p, 3), "tuning": tuning,
             "results_identical": bool(same)}
 
