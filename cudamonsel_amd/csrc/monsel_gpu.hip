@@ -2431,6 +2431,9 @@ constexpr size_t kSlotKeepBytes = (size_t)64 << 20;
 
 void release_job(monsel_ctx_s* C, Job& job) {
   if (job.slot >= 0 && C->slotbuf[job.slot].n > kSlotKeepBytes) C->slotbuf[job.slot].reset();
+  /* its pinned centre staging likewise (the job's copies have ended: it
+   * completed, or its setup failed and the stream was synchronised) */
+  if (job.slot >= 0 && C->pin_cen[job.slot].n > kSlotKeepBytes) C->pin_cen[job.slot].reset();
   C->free_slots.push_back(job.slot);
   job.slot = -1;
   job.base = nullptr;
