@@ -336,11 +336,16 @@ class Engine:
         _check(self.L.monsel_job_stats_get(self.ctx, ctypes.c_uint64(job_id), ctypes.byref(st)), "monsel_job_stats_get")
         r["stats"] = {f: getattr(st, f) for f, _ in JobStats._fields_}
         if nlog > 0:
-            buf = np.zeros(nlog, DETECTED_DTYPE)
+            # the count first, then a buffer for the records kept
             nd = ctypes.c_int64(0)
-            _check(self.L.monsel_job_read_detected(self.ctx, ctypes.c_uint64(job_id), buf.ctypes.data_as(ctypes.c_void_p),
-                                                   ctypes.c_int64(nlog), ctypes.byref(nd)), "monsel_job_read_detected")
-            r["detected"] = buf[:min(nlog, nd.value)]
+            _check(self.L.monsel_job_read_detected(self.ctx, ctypes.c_uint64(job_id), None, ctypes.c_int64(0),
+                                                   ctypes.byref(nd)), "monsel_job_read_detected")
+            buf = np.zeros(min(nlog, nd.value), DETECTED_DTYPE)
+            if len(buf):
+                _check(self.L.monsel_job_read_detected(self.ctx, ctypes.c_uint64(job_id),
+                                                       buf.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(len(buf)),
+                                                       ctypes.byref(nd)), "monsel_job_read_detected")
+            r["detected"] = buf
             r["n_detected"] = nd.value
         self.L.monsel_job_release(self.ctx, ctypes.c_uint64(job_id))
         return r

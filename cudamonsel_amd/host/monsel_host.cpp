@@ -972,9 +972,12 @@ PixelResults MonteCarloSS::runPixels(const std::vector<std::array<double, 3>>& c
     if (st == MONSEL_OK) st = monsel_job_read(dev.ctx(), jid, &t);
     if (st == MONSEL_OK) st = monsel_job_stats_get(dev.ctx(), jid, &pr.stats);
     if (st == MONSEL_OK) {
-      detected.resize((size_t)logCap);
-      st = monsel_job_read_detected(dev.ctx(), jid, detected.data(), logCap, &nDetected);
-      detected.resize((size_t)std::min<int64_t>(nDetected, logCap));
+      /* the count first, then a buffer for the records kept */
+      st = monsel_job_read_detected(dev.ctx(), jid, nullptr, 0, &nDetected);
+      if (st == MONSEL_OK && nDetected > 0) {
+        detected.resize((size_t)std::min<int64_t>(nDetected, logCap));
+        st = monsel_job_read_detected(dev.ctx(), jid, detected.data(), (int64_t)detected.size(), &nDetected);
+      }
     }
     std::string err = st == MONSEL_OK ? std::string() : monsel_last_error();
     monsel_job_release(dev.ctx(), jid);
