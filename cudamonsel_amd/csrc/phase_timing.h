@@ -57,7 +57,7 @@ constexpr int kStepBuckets = 24;
 __device__ unsigned long long g_step_hist[kStepBuckets];
 /* (start, end) of every wave of the launches since the last read, in the
  * constant-rate wall clock (wall_clock64, 100 MHz) */
-constexpr int kMaxSpans = 1 << 16;
+constexpr int kMaxSpans = 1 << 18;
 __device__ unsigned int g_span_count;
 __device__ unsigned long long g_spans[2 * kMaxSpans];
 

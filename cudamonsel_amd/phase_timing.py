@@ -7,6 +7,7 @@ counters and prints, per phase, its share of the wave cycles of the whole
 loop, the cycles per entry and the lane utilisation on entry.
 
 Usage: python -m cudamonsel_amd.phase_timing [per_pixel] [repeats] [--default-engine]
+       python -m cudamonsel_amd.phase_timing --batch   (the 512-job batch)
 (the engine runs the line-scan tuning of data/tuning_linescan.json unless
 --default-engine is given)
 """
@@ -31,7 +32,7 @@ def read_counters(lib, device=0):
     return np.array(out[:], dtype=np.float64).reshape(len(PHASES), 3)
 
 
-def read_spans(lib, device=0, max_spans=1 << 16):
+def read_spans(lib, device=0, max_spans=1 << 18):
     """(step histogram [24], wave spans [n, 2] in 100 MHz ticks) recorded
     since the last read."""
     hist = (ctypes.c_uint64 * 24)()
@@ -114,13 +115,51 @@ def main():
     print(f"{'phase':<10} {'share':>7} {'cyc/entry':>10} {'entries':>12} {'lanes':>6}")
     for name, share, cpe, ent, util in table(c):
         print(f"{name:<10} {share:7.3f} {cpe:10.0f} {ent:12d} {util:6.3f}")
+    report(lib, c)
+
+
+def report(lib, c):
+    """Prints the phase table, the step-count histogram and the launches."""
+    print(f"{'phase':<10} {'share':>7} {'cyc/entry':>10} {'entries':>12} {'lanes':>6}")
+    for name, share, cpe, ent, util in table(c):
+        print(f"{name:<10} {share:7.3f} {cpe:10.0f} {ent:12d} {util:6.3f}")
     hist, spans = read_spans(lib)
     print("steps per electron and launch (log2 buckets):",
           " ".join(f"{k}:{int(v)}" for k, v in enumerate(hist) if v))
+    ls = launches(spans)
+    tot = sum(ms for _, ms, _, _ in ls)
+    if tot > 0:
+        print(f"{len(ls)} launches, {tot:.1f} ms, time-weighted occupancy "
+              f"{sum(ms * occ for _, ms, occ, _ in ls) / tot:.3f}")
     print(f"{'launch':>6} {'waves':>6} {'ms':>8} {'occupancy':>9}  wave ends at (50/90/99/100 %)")
-    for k, (nw, ms, occ, q) in enumerate(launches(spans)):
+    for k, (nw, ms, occ, q) in enumerate(ls):
         print(f"{k:6d} {nw:6d} {ms:8.3f} {occ:9.3f}  " + " ".join(f"{x:.2f}" for x in q))
 
 
+def batch_main():
+    """The 512-job batch workload on its tuning (data/tuning_batch.json)."""
+    import json
+    import os
+    import cudamonsel_amd as M
+    from cudamonsel_amd import serving
+    from cudamonsel_amd.abi import Engine
+    tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "tuning_batch.json")
+    tuning = json.load(open(tfile))["tuning"] if os.path.exists(tfile) else {}
+    print(f"engine tuning: {tuning or 'defaults'}")
+    jobs = serving.batch_workload(512, seed=4)
+    lib = M.gpu_lib()
+    with Engine(**tuning) as eng:
+        serving.run_batch(eng, jobs[:8])   # warm-up
+        read_counters(lib)
+        read_spans(lib)
+        res, t = serving.run_batch(eng, jobs)
+    c = read_counters(lib)
+    print(f"batch of 512 jobs: {sum(r['stats']['steps'] for r in res)} electron steps in {t:.3f} s")
+    report(lib, c)
+
+
 if __name__ == "__main__":
-    main()
+    if "--batch" in sys.argv:
+        batch_main()
+    else:
+        main()
