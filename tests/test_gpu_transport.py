@@ -9,7 +9,8 @@ share that level: each must pass p > 0.01/3, Bonferroni).  Workloads are reduced
 Si at 1 keV with 200 energy bins over 0-1 keV (the bulk-Si configuration),
 a PMMA line on Si at 0.5 keV (the line-scan configuration) in both
 reference-compat and fixed modes, PMMA at 0.5 keV, the tabulated-inelastic
-Si path on synthetic tables, a contact hole, and Cu."""
+Si path on synthetic tables, a contact hole, Cu, and bulk Si at 25 keV
+(the screened-Rutherford elastic branch)."""
 import numpy as np
 import pytest
 
@@ -70,6 +71,20 @@ def test_bulk_si_1keV_200_bins(oracle, M):
     r, o = run_both(oracle, M, sc, 1000.0, centers, 1250, 200)
     check_yields(r, o)
     check_histograms(r, o)
+
+
+def test_bulk_si_25keV(oracle, M):
+    """Bulk Si at 25 keV, 4000 primaries: above 20 keV the elastic
+    cross-sections and angles are the screened Rutherford ones
+    (ScreenedRutherfordScatteringAngle.cu:24-39), so this runs that branch end
+    to end (about 20,000 steps per primary, most of them secondaries)."""
+    from cudamonsel_amd import scenes
+    sc = scenes.flat("Si", beam_eV=25000.0)
+    centers = np.array([[0.0, 0.0, sc.beam_z]] * 16)
+    r, o = run_both(oracle, M, sc, 25000.0, centers, 250, 50)
+    check_yields(r, o)
+    check_histograms(r, o)
+    assert r.stats.steps / o["stats"]["steps"] == pytest.approx(1.0, rel=0.02)
 
 
 @pytest.mark.parametrize("compat", [True, False], ids=["compat", "fixed"])
