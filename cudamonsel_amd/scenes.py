@@ -52,10 +52,11 @@ def vacuum(sc):
     return msm
 
 
-def pmma(sc, min_e_eV=None, barrier_lambda_m=None):
+def pmma(sc, min_e_eV=None, barrier_lambda_m=None, elastic_factory=None):
     """PMMA exactly as LinesOnLayers.cu:86-119; barrier_lambda_m selects the
     quantum-mechanical ExpQMBarrierSM(mat, lambda) instead of the classical
-    one."""
+    one; elastic_factory another SelectableElasticSM factory than
+    NISTMottRS.Factory."""
     breakE = M.ToSI.eV(45.0)
     density = 1190.0
     workfun, bandgap = 5.5, 5.0
@@ -68,7 +69,7 @@ def pmma(sc, min_e_eV=None, barrier_lambda_m=None):
     mat.setWorkfunction(M.ToSI.eV(workfun))
     mat.setBandgap(M.ToSI.eV(bandgap))
     mat.setEnergyCBbottom(M.ToSI.eV(potU))
-    el = M.SelectableElasticSM(mat, M.NISTMottRS.Factory)
+    el = M.SelectableElasticSM(mat, elastic_factory or M.NISTMottRS.Factory)
     csd = M.JoyLuoNieminenCSD(mat, breakE)
     fit = M.FittedInelSM(mat, M.ToSI.eV(65.4), csd)
     pol = M.GanachaudMokraniPolaronTrapSM(2.0e7, 1.0 / M.ToSI.eV(4.0))

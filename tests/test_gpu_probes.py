@@ -279,3 +279,40 @@ def test_barrier(oracle, M, compat, lam):
     same = (g["region"] == o["region"]) & (g["sign"] == o["sign"])
     close = (np.max(np.abs(g["dir"] - o["dir"]), axis=1) < 2e-4) & (np.abs(g["E"] - o["E"]) < 1e-5 * E + 1e-6)
     assert np.mean(same & close) >= 0.999, np.mean(same & close)
+
+
+@pytest.mark.parametrize("factory", ["NISTMottRS.Factory100", "NISTMottRS.Factory100Lin",
+                                     "BrowningEmpiricalCrossSection.Factory",
+                                     "ScreenedRutherfordScatteringAngle.Factory"])
+def test_elastic_factories(oracle, M, factory):
+    """PMMA's elastic mechanism built with each of the reference's other
+    SelectableElasticSM factories (NISTMottRS.cu:16-60: the Mott tables with
+    Browning or linear extrapolation below 100 eV instead of 50 eV; Browning
+    or screened Rutherford alone): mechanism rates from 0.06 eV to 50 keV and
+    every element's angles from 1 eV to 50 keV against the oracle."""
+    from cudamonsel_amd import scenes
+    from cudamonsel_amd.abi import probe_for
+    cls, attr = factory.split(".")
+    f = getattr(getattr(M, cls), attr)
+    sc = scenes.Scene()
+    ch = scenes._chamber(sc)
+    msm = scenes.pmma(sc, elastic_factory=f)
+    shape = scenes._half_space(0.0)
+    reg = M.Region(ch, msm, shape)
+    sc.hold(shape, reg)
+    scenes._finish(sc, 500.0, 0.5, 0.0)
+    pr, cs = probe_for(sc)
+    osc = oracle.OracleScene(cs)
+    E = np.geomspace(0.06, 50000.0, 500).astype(np.float32).astype(np.float64)
+    g = pr.rates(1, E, 3).astype(np.float64)
+    o = osc.rates(1, E, 3)
+    scale = np.maximum(np.abs(o), 1e-30)
+    assert np.all((np.abs(g - o) / scale < 1e-4) | (np.abs(g - o) < 1e-9)), np.max(np.abs(g - o) / scale)
+    rng = np.random.default_rng(21)
+    n = 20000
+    E = np.geomspace(1.0, 50000.0, n).astype(np.float32).astype(np.float64)
+    for slot in range(3):
+        u = rng.random((n, 2)).astype(np.float32).astype(np.float64)
+        ga = pr.elastic_angle(1, slot, E, u).astype(np.float64)
+        oa = osc.elastic_angle(1, slot, E, u)
+        assert np.max(np.abs(np.cos(ga) - np.cos(oa))) < 5e-5, (slot, np.max(np.abs(np.cos(ga) - np.cos(oa))))
