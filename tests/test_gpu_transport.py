@@ -193,3 +193,40 @@ def test_no_stuck_electrons_at_line_corners(M):
         for seed in (1, 2):
             r = eng.result(eng.submit(sid, cen, 307, 377.0, 0.5e-9, seed, max_steps=200000), timeout_s=60)
             assert r["stats"]["watchdog_drops"] == 0 and r["stats"]["queue_overflow"] == 0
+
+
+def test_step_watchdog_against_oracle(oracle, M):
+    """The per-electron step limit (monsel_job_desc.max_steps) at 40 steps:
+    an electron is dropped by the step that takes it past the limit, so no
+    electron takes more than 41 steps; the GPU and the oracle drop the same
+    fraction of electrons and keep the same yields within their statistics."""
+    import math
+    from cudamonsel_amd import scenes
+    from cudamonsel_amd.abi import Engine
+    sc = scenes.flat("PMMA", beam_eV=500.0)
+    cen = np.array([[0.0, 0.0, sc.beam_z]] * 16)
+    per, lim = 500, 40
+    with Engine() as eng:
+        r = eng.result(eng.submit(eng.upload(sc.monte.compileScene()), cen, per, 500.0, 0.5e-9, 13, 50,
+                                  max_steps=lim))
+    osc = oracle.OracleScene(sc.monte.compileScene())
+    o = osc.run(cen, per, M.ToSI.eV(500.0), 0.5e-9, seed=13, nbins=50, threads=0, max_steps=lim)
+    g, c = r["stats"], o["stats"]
+    assert g["watchdog_drops"] > 0 and c["watchdog_drops"] > 0
+    assert g["steps"] <= (lim + 1) * g["electrons"]
+    # drop fractions of the electrons that took a step (binomial)
+    pg, po = g["watchdog_drops"] / g["electrons"], c["watchdog_drops"] / c["electrons"]
+    s = math.sqrt(pg * (1 - pg) / g["electrons"] + po * (1 - po) / c["electrons"])
+    assert abs(pg - po) < 4 * s, (pg, po, s)
+    # SE and BSE yields (BSE = back-escapes that are not SE), with each run's
+    # variance from its per-primary second moments
+    n = 16 * per
+
+    def yields(back, se, se_sq, bse_sq):
+        ys, yb = float(np.sum(se)) / n, float(np.sum(back) - np.sum(se)) / n
+        return ys, yb, (float(np.sum(se_sq)) / n - ys * ys) / n, (float(np.sum(bse_sq)) / n - yb * yb) / n
+
+    gs, gb, gvs, gvb = yields(r["back"], r["se"], r["se_sq"], r["bse_sq"])
+    os_, ob, ovs, ovb = yields(o["back"], o["se"], o["seSq"], o["bseSq"])
+    assert abs(gs - os_) < 3 * math.sqrt(gvs + ovs), (gs, os_)
+    assert abs(gb - ob) < 3 * math.sqrt(gvb + ovb), (gb, ob)
