@@ -230,3 +230,27 @@ def test_step_watchdog_against_oracle(oracle, M):
     os_, ob, ovs, ovb = yields(o["back"], o["se"], o["seSq"], o["bseSq"])
     assert abs(gs - os_) < 3 * math.sqrt(gvs + ovs), (gs, os_)
     assert abs(gb - ob) < 3 * math.sqrt(gvb + ovb), (gb, ob)
+
+
+def test_min_energy_for_tracking(oracle, M):
+    """MONSEL_MaterialScatterModel::setMinEforTracking at 20 eV on PMMA:
+    electrons below it end, and secondaries born below it are not tracked
+    (MonteCarloSS.cu:165-174).  GPU against the oracle, and fewer SEs than
+    without the limit."""
+    from cudamonsel_amd import scenes
+    sc = scenes.Scene()
+    ch = scenes._chamber(sc)
+    msm = scenes.pmma(sc, min_e_eV=20.0)
+    shape = scenes._half_space(0.0)
+    reg = M.Region(ch, msm, shape)
+    sc.hold(shape, reg)
+    scenes._finish(sc, 500.0, 0.5, 0.0)
+    sc.monte.setRefCompat(M.MONSEL_COMPAT_REFERENCE)
+    centers = np.array([[0.0, 0.0, sc.beam_z]] * 16)
+    r, o = run_both(oracle, M, sc, 500.0, centers, 1000, 50)
+    g, _ = check_yields(r, o)
+    check_histograms(r, o)
+    free = scenes.flat("PMMA", beam_eV=500.0)
+    free.monte.setSeed(11)
+    f = free.monte.runPixels(centers, 1000, 50, 50.0)
+    assert np.sum(r.se) < 0.9 * np.sum(f.se)
