@@ -245,3 +245,27 @@ def test_scan_split_by_pixel_range_equals_whole(M):
         assert np.array_equal(sum(p[k] for p in parts), whole[k]), k
     for k in ("primaries", "steps", "electrons", "escapes"):
         assert sum(p["stats"][k] for p in parts) == whole["stats"][k], k
+
+
+def test_jobs_of_both_reference_modes_in_one_context(M):
+    """Scenes of the reference-compatible and the fixed mode cannot share a
+    batch scene: jobs submitted alternately in the two modes run in turns
+    (a job of the other mode waits until the batch in flight drains), and
+    each equals its run alone."""
+    from cudamonsel_amd import scenes
+    from cudamonsel_amd.abi import Engine
+    cen = scenes.scan_centers(np.linspace(-15, 15, 8), [0.0], 0.0)
+    specs = []
+    for k, compat in enumerate((True, False, True, False, False, True)):
+        sc = scenes.line_on_substrate(compat=compat)
+        c = cen.copy()
+        c[:, 2] = sc.beam_z
+        specs.append((sc, sc.monte.compileScene(), c, 120 + 10 * k, 50 + k))
+    with Engine() as eng:
+        sids = [eng.upload(cs) for (_, cs, _, _, _) in specs]
+        ids = [eng.submit(sid, c, per, 500.0, 0.5e-9, seed) for sid, (_, _, c, per, seed) in zip(sids, specs)]
+        mixed = [eng.result(i, timeout_s=120) for i in ids]
+        alone = [eng.result(eng.submit(sid, c, per, 500.0, 0.5e-9, seed))
+                 for sid, (_, _, c, per, seed) in zip(sids, specs)]
+    for a, b in zip(alone, mixed):
+        _same(a, b)
