@@ -163,3 +163,24 @@ def test_scene_beyond_kernel_limits_is_refused(M):
         r = eng.result(eng.submit(eng.upload(ok.monte.compileScene()), [[0.0, 0.0, ok.beam_z]], 100, 1000.0,
                                   0.5e-9, 9))
         assert r["stats"]["primaries"] == 100
+
+
+def test_destroy_with_jobs_in_flight(M):
+    """monsel_destroy with jobs still queued and running: the dispatcher
+    finishes them before it stops, destroy returns MONSEL_OK, and a new
+    context on the device works."""
+    from cudamonsel_amd import scenes
+    from cudamonsel_amd.abi import Engine
+    sc = scenes.flat("PMMA", beam_eV=500.0)
+    cs = sc.monte.compileScene()
+    cen = scenes.scan_centers(np.arange(-8.0, 8.0, 1.0), [0.0], sc.beam_z)
+    eng = Engine()
+    sid = eng.upload(cs)
+    for k in range(4):
+        eng.submit(sid, cen, 5000, 500.0, 0.5e-9, 60 + k)
+    ctx = eng.ctx
+    eng.ctx = ctypes.c_void_p()
+    assert eng.L.monsel_destroy(ctx) == OK
+    with Engine() as again:
+        r = again.result(again.submit(again.upload(cs), cen[:1], 100, 500.0, 0.5e-9, 70))
+        assert r["stats"]["primaries"] == 100
