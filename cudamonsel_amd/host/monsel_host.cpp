@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <fstream>
 #include <functional>
 #include <mutex>
@@ -595,6 +596,38 @@ void Region::removeSubRegion(Region& r) {
 
 /* ----------------------------------------------------------------- beam -- */
 
+static long g_last_electron = 0;
+
+Electron::Electron(const double initialPos[], double theta, double phi, double kE)
+    : pos_{initialPos[0], initialPos[1], initialPos[2]}, theta_(theta), phi_(phi), energy_(kE) {
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lk(mu);
+  ident_ = ++g_last_electron;
+}
+
+Electron* ElectronGun::createElectron() const {
+  const std::array<double, 3> c = getCenter();
+  return new Electron(c.data(), getBeamEnergy());
+}
+
+/* uniform in (0, 1] from a splitmix64 sequence */
+static double unit_open0(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  x ^= x >> 31;
+  return ((double)(x >> 11) + 1.0) * (1.0 / 9007199254740992.0);
+}
+
+Electron* GaussianBeam::createElectron() const {
+  double p[3] = {center_[0], center_[1], center_[2]};
+  const double r = std::sqrt(-2. * std::log(unit_open0(draws_++))) * width_;
+  const double th = 2.0 * M_PI * unit_open0(draws_++);
+  p[0] += r * std::cos(th);
+  p[1] += r * std::sin(th);
+  return new Electron(p, energy_);
+}
+
 GaussianBeam::GaussianBeam(double width) : width_(width), center_{0., 0., -0.99 * MonteCarloSS::ChamberRadius} {}
 
 GaussianBeam::GaussianBeam(double width, double energy, const double c[])
@@ -870,8 +903,8 @@ monsel_scene_id Device::upload(const CompiledScene& s) {
 
 /* --------------------------------------------------------- MonteCarloSS -- */
 
-MonteCarloSS::MonteCarloSS(const ElectronGun* gun, Region* chamber, void* electron) : gun_(gun), chamber_(chamber) {
-  (void)electron;
+MonteCarloSS::MonteCarloSS(const ElectronGun* gun, Region* chamber, Electron* electron)
+    : gun_(gun), chamber_(chamber), electron_(electron) {
   if (!gun_ || !chamber_) throw Error(MONSEL_E_INVALID, "MonteCarloSS: gun and chamber are required");
 }
 

@@ -415,10 +415,41 @@ class Region {
   std::vector<Region*> children_;
 };
 
+/* ---- an electron on the host (NISTMonte/Electron.cu:15-60): a start
+ * point, direction (theta from +z, phi) and kinetic energy.  Transport
+ * creates and moves its electrons on the device; this class is the value
+ * that ElectronGun::createElectron returns and MonteCarloSS's constructor
+ * takes, as in the reference's drivers. */
+class Electron {
+ public:
+  Electron(const double initialPos[], double kE) : Electron(initialPos, 0., 0., kE) {}
+  Electron(const double initialPos[], double theta, double phi, double kE);
+  std::array<double, 3> getPosition() const { return pos_; }
+  void setPosition(const double p[]) { pos_ = {p[0], p[1], p[2]}; }
+  double getEnergy() const { return energy_; }
+  void setEnergy(double e) { energy_ = e; }
+  double getTheta() const { return theta_; }
+  double getPhi() const { return phi_; }
+  void setDirection(double theta, double phi) {
+    theta_ = theta;
+    phi_ = phi;
+  }
+  long getStepCount() const { return 0; }
+  long getIdent() const { return ident_; }
+
+ private:
+  std::array<double, 3> pos_;
+  double theta_, phi_, energy_;
+  long ident_;
+};
+
 /* ---- beam (NISTMonte/ElectronGun.cuh, GaussianBeam.cu) */
 class ElectronGun {
  public:
   virtual ~ElectronGun() = default;
+  /* a new electron at the gun (the caller owns it); this base places it at
+   * the centre */
+  virtual Electron* createElectron() const;
   virtual double getBeamEnergy() const = 0;
   virtual void setBeamEnergy(double e) = 0;
   virtual std::array<double, 3> getCenter() const = 0;
@@ -435,10 +466,16 @@ class GaussianBeam : public ElectronGun {
   double getBeamEnergy() const override { return energy_; }
   void setCenter(const double c[]) override { center_ = {c[0], c[1], c[2]}; }
   std::array<double, 3> getCenter() const override { return center_; }
+  /* GaussianBeam.cu:52-65: the centre moved by a radius sqrt(-2 ln u1) *
+   * width at the angle 2 pi u2, the beam energy, along +z (uniforms from a
+   * host generator of the gun's own; device primaries use their Philox
+   * streams) */
+  Electron* createElectron() const override;
 
  private:
   double width_, energy_ = 0.;
   std::array<double, 3> center_;
+  mutable uint64_t draws_ = 0;
 };
 
 /* ---- detectors (Utility/Histogram.cu, NISTMonte/BackscatterStats.cu) */
@@ -578,7 +615,9 @@ class Device {
 class MonteCarloSS {
  public:
   static constexpr double ChamberRadius = 0.1;  /* MonteCarloSS.cu:32 */
-  MonteCarloSS(const ElectronGun* gun, Region* chamber, void* electron = nullptr);
+  /* the initial electron (MonteCarloSS.cu:40-52), owned from here on; the
+   * trajectories themselves start from the gun on the device */
+  MonteCarloSS(const ElectronGun* gun, Region* chamber, Electron* electron = nullptr);
   void addActionListener(ActionListener& l);
   void removeActionListener(ActionListener& l);
   /* runs n trajectories at the gun's current centre on the GPU */
@@ -606,6 +645,7 @@ class MonteCarloSS {
  private:
   const ElectronGun* gun_;
   Region* chamber_;
+  std::unique_ptr<Electron> electron_;
   std::vector<ActionListener*> listeners_;
   uint32_t compat_ = MONSEL_COMPAT_REFERENCE;
   uint64_t seed_ = 0x5EED5EEDull;

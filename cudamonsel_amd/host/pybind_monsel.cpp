@@ -183,7 +183,20 @@ PYBIND11_MODULE(_monsel, m) {
       .def("getSubRegions", &Region::getSubRegions, py::return_value_policy::reference)
       .def("getScatterModel", &Region::getScatterModel, py::return_value_policy::reference);
 
-  py::class_<ElectronGun>(m, "ElectronGun");
+  py::class_<Electron>(m, "Electron")
+      .def(py::init([](const std::vector<double>& p, double kE) { return new Electron(p.data(), kE); }))
+      .def(py::init([](const std::vector<double>& p, double th, double ph, double kE) {
+        return new Electron(p.data(), th, ph, kE);
+      }))
+      .def("getPosition", &Electron::getPosition)
+      .def("getEnergy", &Electron::getEnergy)
+      .def("getTheta", &Electron::getTheta)
+      .def("getPhi", &Electron::getPhi)
+      .def("getStepCount", &Electron::getStepCount)
+      .def("getIdent", &Electron::getIdent);
+
+  py::class_<ElectronGun>(m, "ElectronGun")
+      .def("createElectron", [](const ElectronGun& g) { return std::unique_ptr<Electron>(g.createElectron()); });
   py::class_<GaussianBeam, ElectronGun>(m, "GaussianBeam")
       .def(py::init<double>())
       .def(py::init([](double w, double e, const std::vector<double>& c) { return new GaussianBeam(w, e, c.data()); }))
@@ -278,6 +291,10 @@ PYBIND11_MODULE(_monsel, m) {
   py::class_<MonteCarloSS>(m, "MonteCarloSS")
       .def(py::init([](const ElectronGun* g, Region* c) { return new MonteCarloSS(g, c); }), py::keep_alive<1, 2>(),
            py::keep_alive<1, 3>())
+      .def(py::init([](const ElectronGun* g, Region* c, const Electron& e) {
+             return new MonteCarloSS(g, c, new Electron(e));
+           }),
+           py::keep_alive<1, 2>(), py::keep_alive<1, 3>())
       .def("addActionListener", &MonteCarloSS::addActionListener, py::keep_alive<1, 2>())
       .def("removeActionListener", &MonteCarloSS::removeActionListener)
       .def("runMultipleTrajectories", &MonteCarloSS::runMultipleTrajectories, py::call_guard<py::gil_scoped_release>())

@@ -123,3 +123,28 @@ def test_job_without_device_fails_loudly(M):
     sc = scenes.flat("PMMA")
     with pytest.raises(M.MonselError):
         sc.monte.runPixels(np.array([[0.0, 0.0, sc.beam_z]]), 10)
+
+
+def test_gaussian_beam_create_electron(M):
+    """GaussianBeam::createElectron (GaussianBeam.cu:52-65): at the beam
+    energy, along +z, in the plane of the centre, at a radius that follows
+    Rayleigh(width) and an angle uniform in [0, 2 pi) (KS); MonteCarloSS
+    takes it as its initial electron, as LinesOnLayers.cu:329 passes it."""
+    import numpy as np
+    from scipy import stats
+    w = 2e-9
+    g = M.GaussianBeam(w, M.ToSI.eV(500.0), [1e-9, -3e-9, -5e-8])
+    es = [g.createElectron() for _ in range(4000)]
+    p = np.array([e.getPosition() for e in es])
+    assert all(e.getEnergy() == M.ToSI.eV(500.0) and e.getTheta() == 0.0 and e.getPhi() == 0.0 for e in es)
+    assert np.all(p[:, 2] == -5e-8)
+    dx, dy = p[:, 0] - 1e-9, p[:, 1] + 3e-9
+    r = np.hypot(dx, dy)
+    assert stats.kstest(r, stats.rayleigh(scale=w).cdf).pvalue > 1e-3
+    ang = np.mod(np.arctan2(dy, dx), 2 * np.pi)
+    assert stats.kstest(ang, stats.uniform(0, 2 * np.pi).cdf).pvalue > 1e-3
+    assert len({e.getIdent() for e in es}) == len(es)
+    from cudamonsel_amd import scenes
+    sc = scenes.flat("Si")
+    mc = M.MonteCarloSS(sc.gun, sc.chamber, g.createElectron())
+    assert mc.getSeed() is not None
