@@ -2120,9 +2120,10 @@ struct Job {
   std::chrono::steady_clock::time_point t_submit, t_start;
   /* set by monsel_job_cancel (any thread), acted on by the worker */
   std::atomic<bool> cancel{false};
+  /* its scene, held from submission to the job's end */
+  std::shared_ptr<SceneDev> sd;
   /* engine state (worker thread only) */
   bool cancelling = false;    /* stopped: drains its queued electrons, then ends */
-  std::shared_ptr<SceneDev> sd;
   int slot = -1;
   uint64_t total = 0, launched = 0;
   uint32_t K = 0;             /* queue entries reserved per launched primary */
@@ -2649,20 +2650,11 @@ void worker_main(monsel_ctx_s* C) {
       if (C->stop && C->queue.empty() && C->active.empty()) return;
       while (!C->queue.empty() && (int)(C->active.size() + admitted.size()) < maxActive) {
         auto j = C->queue.front();
-        auto it = C->scenes.find(j->scene);
-        if (it == C->scenes.end()) {
-          C->queue.pop_front();
-          lk.unlock();
-          finish_job(C, j, MONSEL_E_NOTFOUND, "scene released before the job ran");
-          lk.lock();
-          continue;
-        }
         /* one batch scene holds scenes of one compatibility mode: a job of
          * another mode waits until the jobs in flight are done */
-        const uint32_t mode = it->second->host.compat;
+        const uint32_t mode = j->sd->host.compat;
         if (!admitted.empty() && admitted[0]->sd->host.compat != mode) break;
         if (admitted.empty() && !C->active.empty() && C->active[0]->sd->host.compat != mode) break;
-        j->sd = it->second;
         C->queue.pop_front();
         admitted.push_back(j);
       }
@@ -2948,9 +2940,11 @@ int monsel_job_submit(monsel_ctx ctx, monsel_scene_id scene, const monsel_job_de
   if (!ctx || !out) return set_err(MONSEL_E_INVALID, "NULL argument");
   int v = validate_job(job);
   if (v) return v;
-  if (!find_scene(ctx, scene)) return set_err(MONSEL_E_NOTFOUND, "unknown scene");
+  auto sd = find_scene(ctx, scene);
+  if (!sd) return set_err(MONSEL_E_NOTFOUND, "unknown scene");
   auto j = std::make_shared<Job>();
   j->scene = scene;
+  j->sd = sd;   /* the job holds its scene: a scene released meanwhile lives until the job ends */
   j->desc = *job;
   j->centers.assign(job->pixel_center, job->pixel_center + 3 * (size_t)job->n_pixels);
   j->desc.pixel_center = j->centers.data();

@@ -322,6 +322,8 @@ int monsel_ctx_counters(monsel_ctx ctx, uint64_t* launches, uint64_t* steps, dou
 /* Scenes: an immutable, device-resident copy of the object graph and tables. */
 int monsel_scene_footprint(const monsel_scene_desc* desc, size_t* hbm_bytes);
 int monsel_scene_upload(monsel_ctx ctx, const monsel_scene_desc* desc, monsel_scene_id* out);
+/* Drops the context's reference to a scene; jobs already submitted on it
+ * keep it (it is freed when the last of them ends). */
 int monsel_scene_release(monsel_ctx ctx, monsel_scene_id scene);
 
 /* Jobs.  submit enqueues the work on the context's stream and returns;

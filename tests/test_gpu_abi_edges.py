@@ -184,3 +184,29 @@ def test_destroy_with_jobs_in_flight(M):
     with Engine() as again:
         r = again.result(again.submit(again.upload(cs), cen[:1], 100, 500.0, 0.5e-9, 70))
         assert r["stats"]["primaries"] == 100
+
+
+def test_scene_released_while_its_jobs_wait(M):
+    """A scene released right after jobs were submitted on it (some of them
+    still queued behind a large job) stays alive for those jobs: they complete
+    with the results they have when the scene is kept; new submissions on the
+    released id are refused."""
+    from cudamonsel_amd import scenes
+    from cudamonsel_amd.abi import Engine
+    big = scenes.flat("PMMA", beam_eV=500.0)
+    sc = scenes.flat("Si", beam_eV=1000.0)
+    cen_big = scenes.scan_centers(np.arange(-64.0, 64.0, 1.0), [0.0], big.beam_z)
+    cen = [[0.0, 0.0, sc.beam_z]]
+    with Engine(max_jobs_in_flight=1) as eng:
+        ref = eng.result(eng.submit(eng.upload(sc.monte.compileScene()), cen, 2000, 1000.0, 0.5e-9, 81))
+        first = eng.submit(eng.upload(big.monte.compileScene()), cen_big, 4000, 500.0, 0.5e-9, 80)
+        sid = eng.upload(sc.monte.compileScene())
+        ids = [eng.submit(sid, cen, 2000, 1000.0, 0.5e-9, 81) for _ in range(2)]
+        assert eng.L.monsel_scene_release(eng.ctx, ctypes.c_uint64(sid)) == OK
+        jd, keep = _desc(cen)
+        rc, _ = _submit(eng, sid, jd)
+        assert rc == NOTFOUND
+        for i in ids:
+            r = eng.result(i, timeout_s=60)
+            assert np.array_equal(r["back"], ref["back"]) and np.array_equal(r["se"], ref["se"])
+        eng.result(first, timeout_s=60)
