@@ -28,6 +28,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <string>
 #include <thread>
 #include <vector>
@@ -2753,6 +2754,8 @@ int guarded(F&& f) {
     return f();
   } catch (const MonselError& e) {
     return set_err(e.status, e.msg);
+  } catch (const std::bad_alloc&) {
+    return set_err(MONSEL_E_NOMEM, "host memory exhausted");
   } catch (const std::exception& e) {
     return set_err(MONSEL_E_DEVICE, std::string(e.what()) + " (in " + g_hip_call + ")");
   }
@@ -2842,7 +2845,13 @@ int monsel_destroy(monsel_ctx ctx) {
     ctx->stop = true;
   }
   ctx->cv_work.notify_all();
-  if (ctx->worker.joinable()) ctx->worker.join();
+  if (ctx->worker.joinable()) {
+    try {
+      ctx->worker.join();
+    } catch (const std::exception& e) {
+      return set_err(MONSEL_E_DEVICE, std::string("joining the dispatcher: ") + e.what());
+    }
+  }
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   ctx->scenes.clear();
@@ -2942,22 +2951,25 @@ int monsel_job_submit(monsel_ctx ctx, monsel_scene_id scene, const monsel_job_de
   if (v) return v;
   auto sd = find_scene(ctx, scene);
   if (!sd) return set_err(MONSEL_E_NOTFOUND, "unknown scene");
-  auto j = std::make_shared<Job>();
-  j->scene = scene;
-  j->sd = sd;   /* the job holds its scene: a scene released meanwhile lives until the job ends */
-  j->desc = *job;
-  j->centers.assign(job->pixel_center, job->pixel_center + 3 * (size_t)job->n_pixels);
-  j->desc.pixel_center = j->centers.data();
-  j->t_submit = std::chrono::steady_clock::now();
-  {
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    j->id = ctx->next_id++;
-    ctx->jobs[j->id] = j;
-    ctx->queue.push_back(j);
-    *out = j->id;
-  }
-  ctx->cv_work.notify_one();
-  return MONSEL_OK;
+  /* the job record and its copy of the pixel centres can fail to allocate */
+  return guarded([&] {
+    auto j = std::make_shared<Job>();
+    j->scene = scene;
+    j->sd = sd;   /* the job holds its scene: a scene released meanwhile lives until the job ends */
+    j->desc = *job;
+    j->centers.assign(job->pixel_center, job->pixel_center + 3 * (size_t)job->n_pixels);
+    j->desc.pixel_center = j->centers.data();
+    j->t_submit = std::chrono::steady_clock::now();
+    {
+      std::lock_guard<std::mutex> lk(ctx->mu);
+      j->id = ctx->next_id++;
+      ctx->jobs[j->id] = j;
+      ctx->queue.push_back(j);
+      *out = j->id;
+    }
+    ctx->cv_work.notify_one();
+    return MONSEL_OK;
+  });
 }
 
 int monsel_job_poll(monsel_ctx ctx, monsel_job_id job, int* done) {
