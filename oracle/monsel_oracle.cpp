@@ -78,7 +78,7 @@ static double u_neville(const double* f, int offset, int order, double x, double
   int ns = (int)::round(x);
   if (ns < 0) ns = 0;
   else if (ns > order) ns = order;
-  double c[16], d[16];
+  double c[16] = {}, d[16] = {};
   for (int i = 0; i <= order; ++i) c[i] = d[i] = f[offset + i];
   double y = c[ns--];
   double dy = 0;
@@ -145,7 +145,7 @@ static void nu_locate(double x, const double* xs, int n, int order, int* first, 
 /* NULagrangeInterpolation.cu:85-112 */
 static double nu_neville(const double* f, const double* xs, int offset, int nearest, int order, double x) {
   int ns = nearest - offset;
-  double c[16], d[16];
+  double c[16] = {}, d[16] = {};
   for (int i = 0; i <= order; ++i) c[i] = d[i] = f[offset + i];
   double y = c[ns--];
   for (int m = 1; m <= order; m++) {
