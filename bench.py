@@ -239,9 +239,9 @@ def served_leg(duration_s, rate, tuning, light_rate, light_s):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    # 40 scan jobs (about 0.5 s): the 4-in-flight pipeline's fill and drain
-    # are a small part of the timed region
-    ap.add_argument("--steps", type=int, default=40)
+    # 100 scan jobs (about 1.1 s): the 4-in-flight pipeline's fill and drain
+    # are about 1 % of the timed region
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--per-pixel", type=int, default=2000)
     ap.add_argument("--inflight", type=int, default=4, help="line-scan jobs on the device at once")
