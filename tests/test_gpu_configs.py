@@ -81,11 +81,14 @@ def test_line_scan_full(oracle, M, compat):
     assert r.stats.steps / o["stats"]["steps"] == pytest.approx(1.0, rel=0.01)
 
 
-def test_image_subraster_against_oracle(oracle, M):
+@pytest.mark.parametrize("geo_seed", [3, 11])
+def test_image_subraster_against_oracle(oracle, M, geo_seed):
     """32 x 32 pixels (1 nm pitch) of the 256 x 256 image over the middle line
-    of random_lines(seed=3), 300 primaries per pixel at 0.5 keV."""
+    of random_lines(seed) -- seed 3 is the configuration's; seed 11 draws
+    other widths, heights and sidewall angles -- 300 primaries per pixel at
+    0.5 keV."""
     from cudamonsel_amd import scenes
-    sc = scenes.random_lines(seed=3)
+    sc = scenes.random_lines(seed=geo_seed)
     centers = scenes.scan_centers(np.arange(-16.0, 16.0, 1.0), np.arange(-16.0, 16.0, 1.0), sc.beam_z)
     r = _gpu_run(M, sc, centers, 300, 3, 50)
     o = _oracle_run(oracle, M, sc, centers, 300, 500.0, 3, 50)
