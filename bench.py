@@ -197,6 +197,7 @@ def _stream_legs(reqs, duration_s, tuning):
         log(f"{name}: {r['wall_s']:.2f} s, p50 {r['p50_s']:.2f} s, p99 {r['p99_s']:.2f} s")
         out[name] = {k: round(v, 3) if isinstance(v, float) else v for k, v in r.items() if k != "latencies"}
         out[name]["achieved_primaries_per_s"] = out[name].pop("primaries_per_s")
+        out[name]["achieved_electron_steps_per_s"] = out[name].pop("electron_steps_per_s")
     out["images_per_hour_ratio"] = round(out["served"]["images_per_hour"] / out["plain"]["images_per_hour"], 3)
     return out
 

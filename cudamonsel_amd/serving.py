@@ -233,7 +233,8 @@ def per_pixel_check(engine, requests, sample=16):
 def serve(engine, requests, plain=False, hbm_budget=64 << 30, time_scale=1.0, poll_s=0.0005, log=None,
           keep_results=False):
     """Replays the request stream in real time (arrival times divided by
-    time_scale) and returns per-request latencies and totals; log(msg), if
+    time_scale) and returns per-request latencies and totals (requests,
+    primaries and electron steps per second of wall time); log(msg), if
     given, hears of every completed request.  With keep_results the
     returned dict also maps every request id to its tallies ("results")."""
     cache = SceneCache(engine)
@@ -247,6 +248,7 @@ def serve(engine, requests, plain=False, hbm_budget=64 << 30, time_scale=1.0, po
     lat = {}
     kept = {}
     peak = 0
+    steps = 0
     t0 = time.perf_counter()
     done = 0
     while done < len(requests):
@@ -270,6 +272,7 @@ def serve(engine, requests, plain=False, hbm_budget=64 << 30, time_scale=1.0, po
         for j in finished:
             r, need = running.pop(j)
             res = engine.result(j)
+            steps += int(res.get("stats", {}).get("steps", 0))
             if keep_results:
                 kept[r.rid] = res
             adm.release(need)
@@ -288,7 +291,7 @@ def serve(engine, requests, plain=False, hbm_budget=64 << 30, time_scale=1.0, po
     prim = sum(r.primaries for r in requests)
     out = dict(requests=len(requests), wall_s=wall, p50_s=percentile(vals, 50), p99_s=percentile(vals, 99),
                images_per_hour=3600.0 * len(requests) / wall, primaries=prim, primaries_per_s=prim / wall,
-               latencies=vals, max_in_flight=peak)
+               electron_steps=steps, electron_steps_per_s=steps / wall, latencies=vals, max_in_flight=peak)
     if keep_results:
         out["results"] = kept
     return out
