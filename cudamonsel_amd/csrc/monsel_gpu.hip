@@ -25,6 +25,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <exception>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -61,11 +62,28 @@ static int set_err(int st, const std::string& m) {
       throw MonselError(MONSEL_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e));                \
   } while (0)
 
-struct MonselError {
+struct MonselError : std::exception {
   int status;
   std::string msg;
   MonselError(int s, std::string m) : status(s), msg(std::move(m)) {}
+  const char* what() const noexcept override { return msg.c_str(); }
 };
+
+/* the C-ABI status and message for an exception: a MonselError's own, an
+ * allocation failure as MONSEL_E_NOMEM, anything else as a device error
+ * named by the HIP call in progress */
+static void status_of(const std::exception& ex, int& st, std::string& msg) {
+  if (auto* me = dynamic_cast<const MonselError*>(&ex)) {
+    st = me->status;
+    msg = me->msg;
+  } else if (dynamic_cast<const std::bad_alloc*>(&ex)) {
+    st = MONSEL_E_NOMEM;
+    msg = "host memory exhausted";
+  } else {
+    st = MONSEL_E_DEVICE;
+    msg = std::string(ex.what()) + " (in " + g_hip_call + ")";
+  }
+}
 
 /* ===================================================================== */
 /* device kernels                                                         */
@@ -2267,7 +2285,13 @@ void setup_job(monsel_ctx_s* C, Job& job) {
     if (C->batch_parts[k] == job.sd) part = (int)k;
   if (part < 0) {
     C->batch_parts.push_back(job.sd);
-    rebuild_batch_scene(C);
+    try {
+      rebuild_batch_scene(C);
+    } catch (...) {
+      /* the batch scene on the device is still the one without it */
+      C->batch_parts.pop_back();
+      throw;
+    }
     part = (int)C->batch_parts.size() - 1;
   }
   job.slot = C->free_slots.back();
@@ -2432,11 +2456,13 @@ void complete_collect(monsel_ctx_s* C, Job& job, const char* src) {
 constexpr size_t kSlotKeepBytes = (size_t)64 << 20;
 
 void release_job(monsel_ctx_s* C, Job& job) {
-  if (job.slot >= 0 && C->slotbuf[job.slot].n > kSlotKeepBytes) C->slotbuf[job.slot].reset();
-  /* its pinned centre staging likewise (the job's copies have ended: it
-   * completed, or its setup failed and the stream was synchronised) */
-  if (job.slot >= 0 && C->pin_cen[job.slot].n > kSlotKeepBytes) C->pin_cen[job.slot].reset();
-  C->free_slots.push_back(job.slot);
+  if (job.slot >= 0) {
+    if (C->slotbuf[job.slot].n > kSlotKeepBytes) C->slotbuf[job.slot].reset();
+    /* its pinned centre staging likewise (the job's copies have ended: it
+     * completed, or its setup failed and the stream was synchronised) */
+    if (C->pin_cen[job.slot].n > kSlotKeepBytes) C->pin_cen[job.slot].reset();
+    C->free_slots.push_back(job.slot);
+  }
   job.slot = -1;
   job.base = nullptr;
   job.wd.reset();
@@ -2673,12 +2699,15 @@ void worker_main(monsel_ctx_s* C) {
         try {
           setup_job(C, *j);
           C->active.push_back(j);
-        } catch (const MonselError& e) {
+        } catch (const std::exception& ex) {
           /* copies it enqueued read the slot's pinned staging: they end
            * before the slot is given to another job */
+          int st;
+          std::string msg;
+          status_of(ex, st, msg);
           (void)hipStreamSynchronize(C->stream);
-          if (j->slot >= 0) release_job(C, *j);
-          finish_job(C, j, e.status, e.msg);
+          release_job(C, *j);
+          finish_job(C, j, st, msg);
         }
       }
       if (C->active.empty()) continue;
@@ -2721,14 +2750,9 @@ void worker_main(monsel_ctx_s* C) {
       ended.clear();
     } catch (const std::exception& ex) {
       /* a device failure ends every job in flight */
+      int st;
       std::string msg;
-      int st = MONSEL_E_DEVICE;
-      if (auto* me = dynamic_cast<const MonselError*>(&ex)) {
-        msg = me->msg;
-        st = me->status;
-      } else {
-        msg = std::string(ex.what()) + " (in " + g_hip_call + ")";
-      }
+      status_of(ex, st, msg);
       (void)hipStreamSynchronize(C->stream);
       for (auto& j : ended)
         if (j) C->active.push_back(j);
@@ -2752,12 +2776,11 @@ template <class F>
 int guarded(F&& f) {
   try {
     return f();
-  } catch (const MonselError& e) {
-    return set_err(e.status, e.msg);
-  } catch (const std::bad_alloc&) {
-    return set_err(MONSEL_E_NOMEM, "host memory exhausted");
-  } catch (const std::exception& e) {
-    return set_err(MONSEL_E_DEVICE, std::string(e.what()) + " (in " + g_hip_call + ")");
+  } catch (const std::exception& ex) {
+    int st;
+    std::string msg;
+    status_of(ex, st, msg);
+    return set_err(st, msg);
   }
 }
 
