@@ -1870,8 +1870,11 @@ HostScene compile_scene(const monsel_scene_desc* d) {
       DPrim p;
       memset(&p, 0, sizeof(p));
       if (sh.kind == MONSEL_SHAPE_PLANES) {
-        if (sh.n_planes < 1 || sh.n_planes > kMaxPlanes || sh.first_plane < 0 || sh.first_plane + sh.n_planes > d->n_planes)
+        if (sh.n_planes < 1 || sh.first_plane < 0 || sh.first_plane + sh.n_planes > d->n_planes)
           throw MonselError(MONSEL_E_INVALID, "plane range");
+        if (sh.n_planes > kMaxPlanes)
+          throw MonselError(MONSEL_E_UNSUPPORTED, "multi-plane shape with more than " + std::to_string(kMaxPlanes) +
+                                                      " planes");
         p.kind = PRIM_PLANES;
         p.first_plane = (int)H.planes.size();
         p.n_planes = sh.n_planes;

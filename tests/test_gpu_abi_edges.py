@@ -1,8 +1,9 @@
 """The C-ABI's edge cases and error behaviour (include/monsel_gpu.h): every
 malformed job descriptor is refused with MONSEL_E_INVALID before it is
 queued, unknown ids give MONSEL_E_NOTFOUND, a running job reads as
-MONSEL_E_NOTREADY and a zero timeout as MONSEL_E_TIMEOUT, a scene beyond the
-kernels' limits is refused at upload with MONSEL_E_UNSUPPORTED, and the
+MONSEL_E_NOTREADY and a zero timeout as MONSEL_E_TIMEOUT, scenes beyond the
+kernels' limits (convex parts per shape, planes per multi-plane shape) are
+refused at upload with MONSEL_E_UNSUPPORTED, and the
 smallest and the largest histogram shapes give consistent tallies (the
 histograms go through LDS when they fit and global atomics when they do
 not)."""
@@ -131,6 +132,56 @@ def test_histogram_shapes(flat_pmma, nbins):
     assert int(np.sum(r["elevation_hist"])) == r["stats"]["escapes"]
     assert int(np.sum(r["azimuth_hist"])) == r["stats"]["escapes"]
     assert back > 0
+
+
+def _upload_status(eng, sc):
+    cs = sc.monte.compileScene()   # owns the descriptor's arrays during the upload
+    sid = ctypes.c_uint64()
+    return eng.L.monsel_scene_upload(eng.ctx, ctypes.c_void_p(cs.desc_address()), ctypes.byref(sid))
+
+
+def test_multiplane_shape_limit(M, oracle):
+    """A 17-sided prism (a multi-plane shape of 17 planes; the kernels take
+    16) is refused at upload with MONSEL_E_UNSUPPORTED; the 16-sided one,
+    the largest primitive the kernels take (a PMMA prism of 20 nm inradius
+    and 50 nm height under the beam), agrees with the oracle: BSE and SE
+    yields within 3 sigma, exit histograms by KS."""
+    import stats_util as su
+    from cudamonsel_amd import scenes
+    from cudamonsel_amd.abi import Engine, last_error
+    NM = 1e-9
+
+    def prism(sides):
+        sc = scenes.Scene()
+        ch = scenes._chamber(sc)
+        msm = scenes.MATERIALS["PMMA"](sc)
+        s = M.NormalMultiPlaneShape()
+        for k in range(sides - 2):
+            a = 2.0 * np.pi * k / (sides - 2)
+            s.addPlane([np.cos(a), np.sin(a), 0.0], [20 * NM * np.cos(a), 20 * NM * np.sin(a), 0.0])
+        s.addPlane([0.0, 0.0, -1.0], [0.0, 0.0, 0.0])        # z >= 0, under the beam
+        s.addPlane([0.0, 0.0, 1.0], [0.0, 0.0, 50 * NM])     # z <= 50 nm
+        assert s.getNumPlanes() == sides
+        reg = M.Region(ch, msm, s)
+        sc.hold(s, reg)
+        scenes._finish(sc, 500.0, 0.5, 0.0)
+        return sc
+
+    with Engine() as eng:
+        assert _upload_status(eng, prism(17)) == UNSUPPORTED, last_error()
+        assert "16 planes" in last_error()
+    sc = prism(16)
+    centers = np.array([[x * NM, 0.0, sc.beam_z] for x in np.linspace(-24.0, 24.0, 16)])
+    sc.monte.setSeed(12)
+    r = sc.monte.runPixels(centers, 500, 50, 50.0)
+    o = oracle.OracleScene(sc.monte.compileScene()).run(centers, 500, M.ToSI.eV(500.0), 0.5e-9, seed=12, nbins=50,
+                                                        se_cutoff_eV=50.0, threads=0)
+    assert r.stats.queue_overflow == 0 and r.stats.watchdog_drops == 0 and o["stats"]["watchdog_drops"] == 0
+    zs, zb = su.total_z(su.from_gpu(r), su.from_oracle(o))
+    assert abs(zs) < 3 and abs(zb) < 3, (zs, zb)
+    for name in ("backEnergyHist", "elevationHist", "azimuthHist"):
+        D, p = su.ks_binned(getattr(r, name), o[name])
+        assert p > 0.01 / 3, (name, D, p)
 
 
 def test_scene_beyond_kernel_limits_is_refused(M):
