@@ -3,6 +3,8 @@
   libmonsel_gpu.so   hipcc, gfx950: kernels + the C-ABI of include/monsel_gpu.h
   libmonsel_host.so  g++: the reference-mirror host API (host/monsel_host.cpp)
   _monsel*.so        g++ + pybind11: Python bindings of the host API
+  lines_on_layers    g++: a LinesOnLayers-style C++ driver on the host API
+                     (host/lines_on_layers.cpp)
 
 Usage: python -m cudamonsel_amd.build   (or build() from Python)
        python -m cudamonsel_amd.build --phase-timing
@@ -65,7 +67,10 @@ def build(verbose=True, phase_timing=False):
     _run(["g++", *CXX_FLAGS, "-I" + pybind11.get_include(), "-I" + py_inc,
           os.path.join(HERE, "host", "pybind_monsel.cpp"),
           "-L" + HERE, "-lmonsel_host", "-lmonsel_gpu", "-Wl,-rpath,$ORIGIN", "-o", mod], verbose)
-    return [gpu, host, mod]
+    driver = os.path.join(HERE, "lines_on_layers")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", os.path.join(HERE, "host", "lines_on_layers.cpp"),
+          "-L" + HERE, "-lmonsel_host", "-lmonsel_gpu", "-Wl,-rpath,$ORIGIN", "-o", driver], verbose)
+    return [gpu, host, mod, driver]
 
 
 if __name__ == "__main__":
