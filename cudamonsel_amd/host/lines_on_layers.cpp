@@ -15,14 +15,19 @@
  * (E_SEgen 63 eV, breakE 45 eV) in place of TabulatedInelasticSM.
  *
  * Usage: lines_on_layers [--traj N] [--xmax K] [--y Y1,Y2,...] [--seed S]
- *                        [--fixed] [--out FILE]
+ *                        [--fixed] [--batched] [--out FILE]
  *   --traj   trajectories per landing position (100, as the reference)
  *   --xmax   the first K x positions of the reference's grid (all)
  *   --y      y positions in nm (0)
  *   --seed   MonteCarloSS seed (the class default)
  *   --fixed  the corrected geometry and physics (MONSEL_COMPAT_FIXED)
+ *   --batched  the whole scan as one MonteCarloSS::runPixels job (the
+ *            landing positions run concurrently on the GPU) instead of one
+ *            runMultipleTrajectories call per position; the yields are the
+ *            same estimates from other random streams
  *   --out    output text file (output.txt)
  */
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -47,6 +52,7 @@ struct Options {
   bool seeded = false;
   uint64_t seed = 0;
   bool fixed = false;
+  bool batched = false;
   std::string out = "output.txt";
 };
 
@@ -75,6 +81,8 @@ Options parse(int argc, char** argv) {
       o.seed = std::strtoull(next().c_str(), nullptr, 0);
     } else if (a == "--fixed") {
       o.fixed = true;
+    } else if (a == "--batched") {
+      o.batched = true;
     } else if (a == "--out") {
       o.out = next();
     } else {
@@ -220,7 +228,26 @@ int main(int argc, char** argv) {
 
     const int nbins = (int)(beamEeV / binSizeEV);
     const auto t0 = std::chrono::steady_clock::now();
-    for (double ynm : opt.y) {
+    auto emit = [&](double xnm, double ynm, double bsf, double sef) {
+      const std::string row = fmt(beamEeV) + " " + fmt(xnm) + " " + fmt(ynm) + " " + fmt(bsf) + " " + fmt(sef);
+      printf("%s\n", row.c_str());
+      output += "\n" + row;
+    };
+    if (opt.batched) {
+      /* one job over every landing position; per-pixel SE counts use the
+       * same histogram bins below 50 eV as the listener's histogram */
+      std::vector<std::array<double, 3>> centers;
+      for (double ynm : opt.y)
+        for (double xnm : xvals) centers.push_back({xnm * nm, ynm * nm, -h - 20. * nm});
+      PixelResults r = monte.runPixels(centers, opt.traj, nbins, 50.);
+      int p = 0;
+      for (double ynm : opt.y)
+        for (double xnm : xvals) {
+          emit(xnm, ynm, r.bseYield(p), r.seYield(p));
+          ++p;
+        }
+    }
+    for (double ynm : opt.batched ? std::vector<double>{} : opt.y) {
       for (double xnm : xvals) {
         const double c[] = {xnm * nm, ynm * nm, -h - 20. * nm};
         eg.setCenter(c);
@@ -236,9 +263,7 @@ int main(int argc, char** argv) {
         for (int j = 0; j < seBins; ++j) totalSE += hist.counts(j);
         const double sef = (double)totalSE / opt.traj;
         const double bsf = back.backscatterFraction() - sef;
-        const std::string row = fmt(beamEeV) + " " + fmt(xnm) + " " + fmt(ynm) + " " + fmt(bsf) + " " + fmt(sef);
-        printf("%s\n", row.c_str());
-        output += "\n" + row;
+        emit(xnm, ynm, bsf, sef);
         monte.removeActionListener(back);
       }
     }

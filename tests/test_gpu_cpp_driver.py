@@ -80,3 +80,25 @@ def test_driver_rows_agree_with_the_oracle(M, oracle, driver_rows):
     z_se = (g_se - o_se) / np.sqrt(2.0 * var_se / n)
     z_bse = (g_bse - o_bse) / np.sqrt(2.0 * var_bse / n)
     assert abs(z_se) < 3 and abs(z_bse) < 3, (g_se, o_se, z_se, g_bse, o_bse, z_bse)
+
+
+def test_batched_driver_rows_equal_run_pixels(M, tmp_path):
+    """--batched runs the same landing positions as one runPixels job; its
+    rows equal runPixels through the bindings digit for digit."""
+    from cudamonsel_amd import outputs, scenes
+    out = tmp_path / "batched.txt"
+    p = subprocess.run([EXE, "--batched", "--traj", str(TRAJ), "--xmax", str(NX), "--seed", str(SEED), "--out",
+                        str(out)], capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr
+    t = outputs.read_scan_text(str(out))
+    printed = [l for l in p.stdout.splitlines() if l.strip()]
+    assert len(printed) == NX == len(t["x_nm"])
+    sc = scenes.lines_on_layers()
+    sc.monte.setSeed(SEED)
+    centers = [[x * 1e-9, 0.0, -120.0 * 1e-9 - 20.0 * 1e-9] for x in t["x_nm"]]
+    r = sc.monte.runPixels(centers, TRAJ, 50, 50.0)
+    for k, x_nm in enumerate(t["x_nm"]):
+        se = int(r.se[k]) / TRAJ
+        bse = (int(r.back[k]) - int(r.se[k])) / TRAJ
+        row = " ".join("%f" % v for v in (500.0, x_nm, 0.0, bse, se))
+        assert printed[k] == row, (k, printed[k], row)
