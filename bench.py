@@ -154,6 +154,137 @@ def cpu_baseline(sc, centers, target_s=10.0):
             "cores_note": f"one worker per core of the {HOST_CORES_PER_GPU}-core share of one GPU"}
 
 
+def _gpu_timed(eng, sid, centers, per, beam_eV, seed, nbins=50):
+    t0 = time.perf_counter()
+    r = eng.result(eng.submit(sid, centers, per, beam_eV, 0.5e-9, seed, nbins=nbins), timeout_s=600)
+    return r, time.perf_counter() - t0
+
+
+def _oracle_timed(osc, centers, per, beam_eV, seed, nbins, threads):
+    import numpy as np
+    import cudamonsel_amd as M
+    t0 = time.perf_counter()
+    o = osc.run(np.asarray(centers, np.float64), per, M.ToSI.eV(beam_eV), 0.5e-9, seed=seed, nbins=nbins,
+                threads=threads)
+    return o, time.perf_counter() - t0
+
+
+def _side_by_side(gpu_prim, gpu_steps, gpu_s, cpu_prim, cpu_steps, cpu_s):
+    return {"gpu_primaries_per_s": round(gpu_prim / gpu_s, 1), "gpu_electron_steps_per_s": round(gpu_steps / gpu_s, 1),
+            "cpu_primaries_per_s": round(cpu_prim / cpu_s, 1), "cpu_electron_steps_per_s": round(cpu_steps / cpu_s, 1),
+            "gpu_over_cpu": round((gpu_prim / gpu_s) / (cpu_prim / cpu_s), 1)}
+
+
+def cpu_configs_leg():
+    """The oracle (one worker per host core of this GPU's share) beside the
+    GPU on the other workloads, on the same inputs:
+    - bulk Si at 1 keV, 20,000 primaries, TabulatedInelasticSM on the
+      dielectric-model tables: both in full (the oracle's 20,000 primaries as
+      16 pixels at the beam centre, its thread pool being pixel-granular);
+    - the 256 x 256 x 300 image of three seeded lines at 0.5 keV: the oracle
+      on a 32 x 32 sub-raster over the middle line, extrapolated to the image
+      by pixels; the GPU on the same sub-raster and on the whole image;
+    - the 512 single-pixel yield jobs: the GPU batched in full; the oracle on
+      up to 1,000 primaries of each of the first 64 jobs, its electron steps/s
+      extrapolated to the batch's electron steps;
+    - the image-request stream (2 requests/s for 10 minutes): the host cores
+      the oracle would need for its offered load, from the oracle's electron
+      steps/s on 4 x 4 pixels x 20 primaries of each of the stream's first
+      32 requests, and their steps per primary weighted by each request's
+      primaries."""
+    import tempfile
+    import numpy as np
+    from oracle import oracle as O
+    from cudamonsel_amd import scenes, serving
+    from cudamonsel_amd.abi import Engine
+    os.environ.setdefault("MONSEL_TABLE_CACHE", tempfile.mkdtemp(prefix="monsel_tables_"))
+    nproc, _ = host_cpu()
+    threads = min(nproc, HOST_CORES_PER_GPU)
+    out = {"cores": threads}
+    with Engine() as eng:
+        sc = scenes.flat("Si-dielectric", beam_eV=1000.0)
+        sid = eng.upload(sc.monte.compileScene())
+        c = np.array([[0.0, 0.0, sc.beam_z]])
+        _gpu_timed(eng, sid, c, 2000, 1000.0, 0, nbins=200)    # warm-up
+        g, tg = _gpu_timed(eng, sid, c, 20000, 1000.0, 1, nbins=200)
+        o, tc = _oracle_timed(O.OracleScene(sc.monte.compileScene()), np.repeat(c, 16, 0), 1250, 1000.0, 1, 200,
+                              threads)
+        out["bulk_si_1keV"] = dict(_side_by_side(20000, g["stats"]["steps"], tg, 20000, o["stats"]["steps"], tc),
+                                   gpu_s=round(tg, 4), cpu_s=round(tc, 2), primaries=20000)
+        log(f"cpu configs: bulk Si 1 keV, GPU {tg:.3f} s, CPU {tc:.2f} s")
+        sc = scenes.random_lines(seed=3)
+        sid = eng.upload(sc.monte.compileScene())
+        sub = scenes.scan_centers(np.arange(-16.0, 16.0, 1.0), np.arange(-16.0, 16.0, 1.0), sc.beam_z)
+        _gpu_timed(eng, sid, sub[:16], 50, 500.0, 0)            # warm-up
+        g, tg = _gpu_timed(eng, sid, sub, 300, 500.0, 3)
+        full = serving.raster_centers(256, sc.beam_z)
+        gf, tgf = _gpu_timed(eng, sid, full, 300, 500.0, 3)
+        o, tc = _oracle_timed(O.OracleScene(sc.monte.compileScene()), sub, 300, 500.0, 3, 50, threads)
+        n_sub, n_full = sub.shape[0] * 300, full.shape[0] * 300
+        out["image_256x256"] = dict(_side_by_side(n_sub, g["stats"]["steps"], tg, n_sub, o["stats"]["steps"], tc),
+                                    gpu_image_s=round(tgf, 3), gpu_image_primaries_per_s=round(n_full / tgf, 1),
+                                    cpu_image_s_extrapolated=round(tc * n_full / n_sub, 1),
+                                    gpu_image_over_cpu=round((n_full / tgf) / (n_sub / tc), 1),
+                                    sample="32 x 32 pixels x 300 primaries (both); whole image on the GPU")
+        log(f"cpu configs: image, GPU sub-raster {tg:.3f} s, whole image {tgf:.3f} s, CPU sub-raster {tc:.2f} s")
+    tfile = os.path.join(ROOT, "cudamonsel_amd", "data", "tuning_batch.json")
+    tuning = json.load(open(tfile))["tuning"] if os.path.exists(tfile) else {}
+    jobs = serving.batch_workload(512, seed=4)
+    with Engine(**tuning) as eng:
+        serving.run_batch(eng, jobs[:8])
+        res, tb = serving.run_batch(eng, jobs)
+        steps_gpu = sum(int(r["stats"]["steps"]) for r in res)
+        cache = serving.SceneCache(eng)
+        oscs, cpu_steps, cpu_prim, cpu_t = {}, 0, 0, 0.0
+        for j in jobs[:64]:
+            sc, cs, _ = serving._yield_scene(cache, j)
+            key = (j.material, j.step_edge)
+            if key not in oscs:
+                oscs[key] = O.OracleScene(cs)
+            per = max(1, min(j.primaries, 1000) // 16)
+            o, dt = _oracle_timed(oscs[key], [[0.0, 0.0, sc.beam_z]] * 16, per, j.beam_eV, j.seed, 50, threads)
+            cpu_steps += int(o["stats"]["steps"])
+            cpu_prim += 16 * per
+            cpu_t += dt
+    prim = sum(j.primaries for j in jobs)
+    cpu_sps = cpu_steps / cpu_t
+    out["batch_512"] = {"gpu_primaries_per_s": round(prim / tb, 1), "gpu_electron_steps_per_s": round(steps_gpu / tb, 1),
+                        "gpu_s": round(tb, 3), "cpu_electron_steps_per_s": round(cpu_sps, 1),
+                        "cpu_s_extrapolated": round(steps_gpu / cpu_sps, 1),
+                        "gpu_over_cpu": round((steps_gpu / tb) / cpu_sps, 1),
+                        "sample": f"first 64 jobs, up to 1,000 primaries each ({cpu_prim} primaries, {cpu_steps} "
+                                  f"steps in {cpu_t:.1f} s); extrapolated by the batch's {steps_gpu} electron steps"}
+    log(f"cpu configs: batch, GPU {tb:.3f} s, CPU {steps_gpu / cpu_sps:.0f} s extrapolated")
+    reqs = serving.request_stream(rate_per_s=2.0, duration_s=600.0, seed=5)
+    offered = sum(r.primaries for r in reqs) / 600.0
+    with Engine() as eng:
+        cache = serving.SceneCache(eng)
+        oscs, cpu_steps, cpu_prim, cpu_t, w_steps, w_prim = {}, 0, 0, 0.0, 0.0, 0
+        for r in reqs[:32]:
+            sc, cs, _ = serving._request_scene(cache, r)
+            if id(cs) not in oscs:
+                oscs[id(cs)] = O.OracleScene(cs)
+            cen = serving.raster_centers(4, sc.beam_z)
+            o, dt = _oracle_timed(oscs[id(cs)], cen, 20, r.beam_eV, 7 + r.rid, 50, threads)
+            n = cen.shape[0] * 20
+            cpu_steps += int(o["stats"]["steps"])
+            cpu_prim += n
+            cpu_t += dt
+            # the offered load weights each request by its primaries
+            w_steps += r.primaries * int(o["stats"]["steps"]) / n
+            w_prim += r.primaries
+    steps_per_prim = w_steps / w_prim
+    per_core = cpu_steps / cpu_t / threads
+    out["request_stream"] = {"requests": len(reqs), "offered_primaries_per_s": round(offered, 1),
+                             "electron_steps_per_primary": round(steps_per_prim, 1),
+                             "cpu_electron_steps_per_s_per_core": round(per_core, 1),
+                             "cpu_cores_for_offered_load": round(offered * steps_per_prim / per_core, 1),
+                             "sample": f"first 32 requests, 4 x 4 pixels x 20 primaries each ({cpu_prim} primaries, "
+                                       f"{cpu_steps} steps in {cpu_t:.1f} s); steps per primary weighted by "
+                                       f"each request's primaries"}
+    return out
+
+
 def batched_leg(n_jobs, tuning):
     """The batch configuration: n_jobs single-pixel yield jobs (E0 log-uniform
     0.2-5 keV, Si/PMMA/Cu, flat or 20 nm step edge, 1e2-1e5 primaries)
@@ -250,6 +381,8 @@ def main():
                     help="JSON object of engine tuning fields for the line scan "
                          "(default: cudamonsel_amd/data/tuning_linescan.json)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-configs", action="store_true",
+                    help="also time the oracle beside the GPU on the bulk, image, batch and stream workloads")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="run one device copy of this size first (HBM-counter calibration under rocprof)")
     ap.add_argument("--batch-jobs", type=int, default=512, help="jobs of the batched leg (0: skip)")
@@ -355,6 +488,8 @@ def main():
                                    args.serve_light_rate, args.serve_light_seconds)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc, centers)
+    if args.cpu_configs:
+        out["cpu_baseline_other_workloads"] = cpu_configs_leg()
     print(json.dumps(out), flush=True)
 
 
