@@ -293,7 +293,8 @@ typedef struct monsel_tuning {
   int32_t admission_order;      /* new primaries go first to: 0 the oldest job, 1 the job with fewest left */
   int32_t sort_queue;           /* 0: sort each launch's queue by region and energy class; -1: keep spawn order */
   int32_t geometry_batch;       /* lanes of a wave whose step may reach a boundary that wait before the wave
-                                 * runs findEndOfStep for them (0 = default 1 = no waiting, <= 64) */
+                                 * runs findEndOfStep for them (<= 64; 0 = by scene: 8 where a region's own
+                                 * and child shapes hold 3 or more primitives, else 1 = no waiting) */
   int32_t spawn_handoff;        /* 0: a new secondary goes straight to an idle lane of the wave that
                                  * spawned it when there is one; -1: every one waits in the queue
                                  * for the next launch */
