@@ -26,7 +26,8 @@ from .abi import Engine
 SPACE = {
     "block_size": [256, 128],
     "steps_per_launch": [0, 512, 128],
-    "primaries_per_launch": [0, 1 << 20, 1 << 17],
+    # 0 is the engine's default (262,144); 1 << 30 admits all the queue holds
+    "primaries_per_launch": [0, 1 << 20, 1 << 17, 1 << 30],
     "max_jobs_in_flight": [0, 256, 64],
     "tail_requeue": [0, 16, 48, -1],
     "spawn_handoff": [0, -1],
@@ -100,7 +101,7 @@ def load(path):
 # the tail threshold get finer steps (their best values moved with the cost
 # of the between-launch queue sort)
 LINESCAN_SPACE = {
-    "primaries_per_launch": [262144, 131072, 524288, 0],
+    "primaries_per_launch": [262144, 131072, 524288, 1 << 30],
     "steps_per_launch": [0, 512, 256, 128, 64],
     "tail_requeue": [48, 32, 64, 16, -1],
     "geometry_batch": [8, 4, 12, 16],

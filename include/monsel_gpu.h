@@ -288,7 +288,8 @@ typedef struct monsel_tuning {
   int32_t max_generations;   /* transport launches per job before its electrons are dropped */
   int32_t queue_capacity;    /* electrons per generation buffer (0 = from the device memory) */
   int32_t max_jobs_in_flight;   /* jobs sharing the transport launches (0 = 1024) */
-  int32_t primaries_per_launch; /* new primaries admitted per launch (0 = queue capacity) */
+  int32_t primaries_per_launch; /* new primaries admitted per launch (0 = 262,144, or the queue capacity
+                                 * if smaller) */
   int32_t steps_per_launch;     /* regroup interval: steps an electron takes per launch (0 = unlimited) */
   int32_t admission_order;      /* new primaries go first to: 0 the oldest job, 1 the job with fewest left */
   int32_t sort_queue;           /* 0: sort each launch's queue by region and energy class; -1: keep spawn order */
