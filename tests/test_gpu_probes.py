@@ -102,10 +102,16 @@ def test_find_end_of_step(oracle, compat):
         gu, gnx, _ = pr.find_end_of_step(r, p0[m], p1[m])
         same = (onx == gnx)
         hit = (ou <= 1.0)
-        ok = same & (~hit | (np.abs(ou - gu.astype(np.float64)) * L[m] < 1e-3))
+        # crossing distance apart, for segments that cross (u is infinite otherwise)
+        apart = np.zeros(hit.shape)
+        apart[hit] = np.abs(ou[hit] - gu[hit].astype(np.float64)) * L[m][hit]
+        ok = same & (~hit | (apart < 1e-3))
         agree += ok.sum()
         tot += m.sum()
-    assert agree / tot >= 0.995
+    # measured: 20,000 of 20,000 in both modes; the allowance is for FP32
+    # rounding of grazing crossings against the FP64 oracle
+    print(f"findEndOfStep agreement {agree}/{tot}")
+    assert tot - agree <= 10, (agree, tot)
 
 
 @pytest.mark.parametrize("geometry", ["line-compat", "line-fixed", "three-lines", "holes", "step"])
