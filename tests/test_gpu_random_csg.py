@@ -8,7 +8,10 @@ depth-counter walk of the reference's shapes: the region containing a point
 on 20,000 random segments per shape, up to half of them from inside it; and
 electron transport through three of them (yields within 3 sigma, exit
 histograms by KS), which also runs the safety bound that lets steps skip
-findEndOfStep on shapes it was not built around."""
+findEndOfStep on shapes it was not built around.  Nested scenes put a
+random Si region inside a random PMMA one, with a random PMMA sibling beside
+them, so that locating descends a region tree and findEndOfStep tests a
+region's own shape and its children's."""
 import numpy as np
 import pytest
 
@@ -59,6 +62,26 @@ def random_shape(M, rng, n_leaves, hold):
         return s
 
     return build(n_leaves)
+
+
+def nested_csg_scene(M, seed):
+    """chamber -> A (PMMA, 3 parts) -> B (Si, 2 parts); chamber -> C (PMMA,
+    2 parts, 25 nm along x)."""
+    from cudamonsel_amd import scenes
+    rng = np.random.default_rng(seed)
+    sc = scenes.Scene()
+    ch = scenes._chamber(sc)
+    hold = []
+    a = random_shape(M, rng, 3, hold)
+    b = random_shape(M, rng, 2, hold)
+    c = random_shape(M, rng, 2, hold)
+    M.NShapes.translate(c, [25.0 * NM, 0.0, 0.0])
+    ra = M.Region(ch, scenes.MATERIALS["PMMA"](sc), a)
+    rb = M.Region(ra, scenes.MATERIALS["Si"](sc), b)
+    rc = M.Region(ch, scenes.MATERIALS["PMMA"](sc), c)
+    sc.hold(*hold, ra, rb, rc)
+    scenes._finish(sc, 500.0, 0.5, 40.0)
+    return sc
 
 
 def csg_scene(M, seed, n_leaves, compat=True):
@@ -140,3 +163,48 @@ def test_random_csg_transport_against_oracle(oracle, M, seed, n_leaves):
     for name in ("backEnergyHist", "elevationHist", "azimuthHist"):
         D, p = su.ks_binned(getattr(r, name), o[name])
         assert p > 0.01 / 3, (name, D, p)
+
+
+@pytest.mark.parametrize("seed", [3001, 3002, 3003])
+def test_nested_csg_geometry_and_transport_against_oracle(oracle, M, seed):
+    from cudamonsel_amd.abi import probe_for
+    sc = nested_csg_scene(M, seed)
+    pr, cs = probe_for(sc)
+    osc = oracle.OracleScene(cs)
+    rng = np.random.default_rng(seed)
+    cand = rng.uniform(-30.0, 50.0, (400000, 3)) * np.array([1.0, 0.75, 0.75])
+    where = osc.locate(cand)
+    regions = [r for r in np.unique(where) if r > 0]
+    assert len(regions) == 3, regions                  # every region has volume
+    parts = [cand[where == r][:4000] for r in regions]
+    p0 = np.concatenate(parts + [cand[where == 0][:20000 - sum(x.shape[0] for x in parts)]])
+    n = p0.shape[0]
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    L = rng.uniform(0.5, 50.0, n)
+    p1 = p0 + d * L[:, None]
+    p0 = p0.astype(np.float32).astype(np.float64)
+    p1 = p1.astype(np.float32).astype(np.float64)
+    reg = osc.locate(p0)
+    assert np.sum(pr.locate(p0) != reg) <= 20
+    agree = tot = 0
+    for r in np.unique(reg):
+        if r < 0:
+            continue
+        m = reg == r
+        ou, onx, _ = osc.find_end_of_step(r, p0[m], p1[m])
+        gu, gnx, _ = pr.find_end_of_step(r, p0[m], p1[m])
+        hit = ou <= 1.0
+        apart = np.zeros(hit.shape)
+        apart[hit] = np.abs(ou[hit] - gu[hit].astype(np.float64)) * L[m][hit]
+        agree += ((onx == gnx) & (~hit | (apart < 1e-3))).sum()
+        tot += m.sum()
+    print(f"nested {seed}: findEndOfStep agreement {agree}/{tot}")
+    assert tot - agree <= 20, (agree, tot)
+    centers = np.array([[x * NM, 0.0, sc.beam_z] for x in np.linspace(-10.0, 35.0, 16)])
+    sc.monte.setSeed(seed)
+    g = sc.monte.runPixels(centers, 500, 50, 50.0)
+    o = osc.run(centers, 500, M.ToSI.eV(500.0), 0.5e-9, seed=seed, nbins=50, se_cutoff_eV=50.0, threads=0)
+    assert g.stats.queue_overflow == 0 and g.stats.watchdog_drops == 0 and o["stats"]["watchdog_drops"] == 0
+    zs, zb = su.total_z(su.from_gpu(g), su.from_oracle(o))
+    assert abs(zs) < 3 and abs(zb) < 3, (zs, zb)
