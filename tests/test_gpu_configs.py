@@ -16,7 +16,8 @@ themselves where the oracle would take minutes.
   together equal the same jobs run one at a time bit for bit; the first 64
   jobs agree with the oracle job by job.
 - Serving: a shortened Poisson stream served with continuous batching gives
-  every request the tallies it has when run alone.
+  every request the tallies it has when run alone; its request geometries
+  (random line arrays, contact holes) at 0.3, 1 and 2 keV against the oracle.
 
 Reference: LinesOnLayers.cu:351-380 (scan loop, yields), MonteCarloSS.cu:
 222-231 (runMultipleTrajectories), BackscatterStats.cu:140-181 (tallies)."""
@@ -95,6 +96,29 @@ def test_image_subraster_against_oracle(oracle, M, geo_seed):
     assert r.stats.queue_overflow == 0 and r.stats.watchdog_drops == 0 and o["stats"]["watchdog_drops"] == 0
     # 2,048 per-pixel z-scores: max |z| < 4.5 with 99 % probability
     _check_pixels(su.from_gpu(r), su.from_oracle(o), 4.5)
+    _check_hists(r, o)
+
+
+@pytest.mark.parametrize("geometry", ["lines", "holes"])
+@pytest.mark.parametrize("E_eV", [300.0, 1000.0, 2000.0])
+def test_request_geometries_against_oracle(oracle, M, geometry, E_eV):
+    """The serving stream's request geometries at the ends and middle of its
+    0.3-2 keV range: a seeded random-line array (seed 104) and a contact
+    hole (radius 14 nm, depth 70 nm), a 4 x 4 raster at 4 nm pitch across
+    the middle line or the hole's edge, 500 primaries per pixel."""
+    from cudamonsel_amd import scenes
+    if geometry == "lines":
+        sc = scenes.random_lines(seed=104, beam_eV=E_eV)
+    else:
+        sc = scenes.contact_holes(radius_nm=14.0, depth_nm=70.0, beam_eV=E_eV)
+    centers = scenes.scan_centers(np.arange(-6.0, 10.0, 4.0) + (14.0 if geometry == "holes" else 0.0),
+                                  np.arange(-6.0, 10.0, 4.0), sc.beam_z)
+    nbins = int(E_eV / 10.0)       # 10 eV bins, as LinesOnLayers.cu:359 sizes them
+    r = _gpu_run(M, sc, centers, 500, 41, nbins)
+    o = _oracle_run(oracle, M, sc, centers, 500, E_eV, 41, nbins)
+    assert r.stats.queue_overflow == 0 and r.stats.watchdog_drops == 0 and o["stats"]["watchdog_drops"] == 0
+    assert o["se"].sum() > 0
+    _check_pixels(su.from_gpu(r), su.from_oracle(o), 4.0)
     _check_hists(r, o)
 
 
