@@ -243,29 +243,29 @@ int main(int argc, char** argv) {
       int p = 0;
       for (double ynm : opt.y)
         for (double xnm : xvals) {
-          emit(xnm, ynm, r.bseYield(p), r.seYield(p));
-          ++p;
+            emit(xnm, ynm, r.bseYield(p), r.seYield(p));
+            ++p;
         }
-    }
-    for (double ynm : opt.batched ? std::vector<double>{} : opt.y) {
-      for (double xnm : xvals) {
-        const double c[] = {xnm * nm, ynm * nm, -h - 20. * nm};
-        eg.setCenter(c);
-        BackscatterStats back(monte, nbins);
-        monte.addActionListener(back);
-        monte.runMultipleTrajectories(opt.traj);
-        /* SE: back-scattered electrons in the bins below 50 eV; BSE: the
-         * rest of the back-scattered fraction (:360-371) */
-        const Histogram& hist = back.backscatterEnergyHistogram();
-        const double perBin = beamEeV / hist.binCount();
-        const int seBins = (int)(50. / perBin);
-        long totalSE = 0;
-        for (int j = 0; j < seBins; ++j) totalSE += hist.counts(j);
-        const double sef = (double)totalSE / opt.traj;
-        const double bsf = back.backscatterFraction() - sef;
-        emit(xnm, ynm, bsf, sef);
-        monte.removeActionListener(back);
-      }
+    } else {
+      for (double ynm : opt.y)
+        for (double xnm : xvals) {
+          const double c[] = {xnm * nm, ynm * nm, -h - 20. * nm};
+          eg.setCenter(c);
+          BackscatterStats back(monte, nbins);
+          monte.addActionListener(back);
+          monte.runMultipleTrajectories(opt.traj);
+          /* SE: back-scattered electrons in the bins below 50 eV; BSE: the
+           * rest of the back-scattered fraction (:360-371) */
+          const Histogram& hist = back.backscatterEnergyHistogram();
+          const double perBin = beamEeV / hist.binCount();
+          const int seBins = (int)(50. / perBin);
+          long totalSE = 0;
+          for (int j = 0; j < seBins; ++j) totalSE += hist.counts(j);
+          const double sef = (double)totalSE / opt.traj;
+          const double bsf = back.backscatterFraction() - sef;
+          emit(xnm, ynm, bsf, sef);
+          monte.removeActionListener(back);
+        }
     }
     const double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     output += "\n" + std::to_string(elapsed);
