@@ -320,19 +320,44 @@ def batch_timing(engine, jobs):
                          read_after_last_s=wall - last, primaries_per_s=prim / wall)
 
 
+def image_timing(engine, side=256, per_pixel=300, beam_eV=500.0, geo_seed=3, seed=3):
+    """One side x side image of random_lines(geo_seed) at 1 nm pitch as one
+    job, after a warm-up job of 256 pixels x 50 primaries; returns its
+    tallies and the timing."""
+    sc = scenes.random_lines(seed=geo_seed, beam_eV=beam_eV)
+    sid = engine.upload(sc.monte.compileScene())
+    cen = raster_centers(side, sc.beam_z)
+    engine.result(engine.submit(sid, cen[:256], 50, beam_eV, 0.5e-9, seed + 1))
+    t0 = time.perf_counter()
+    r = engine.result(engine.submit(sid, cen, per_pixel, beam_eV, 0.5e-9, seed), timeout_s=600)
+    wall = time.perf_counter() - t0
+    prim = cen.shape[0] * per_pixel
+    return r, dict(pixels=int(cen.shape[0]), primaries=prim, wall_s=wall, primaries_per_s=prim / wall,
+                   electron_steps=int(r["stats"]["steps"]), launches=int(r["stats"]["transport_launches"]))
+
+
 def main(argv=None):
     """python -m cudamonsel_amd.serving batch [--jobs N] [--tuning FILE]:
     the batch workload (N single-pixel yield jobs, seed 4) on a tuned engine,
-    after a warm-up of its first 8 jobs; prints one JSON line."""
+    after a warm-up of its first 8 jobs; prints one JSON line.
+    python -m cudamonsel_amd.serving image [--side N]: the 256 x 256 x 300
+    image of three seeded lines at 0.5 keV as one job on the engine's
+    default settings; prints one JSON line."""
     import argparse
     import json
     import os
     from .abi import Engine
     ap = argparse.ArgumentParser(prog="python -m cudamonsel_amd.serving")
-    ap.add_argument("form", choices=["batch"])
+    ap.add_argument("form", choices=["batch", "image"])
     ap.add_argument("--jobs", type=int, default=512)
+    ap.add_argument("--side", type=int, default=256)
     ap.add_argument("--tuning", default=os.path.join(os.path.dirname(__file__), "data", "tuning_batch.json"))
     a = ap.parse_args(argv)
+    if a.form == "image":
+        with Engine() as eng:
+            _, t = image_timing(eng, side=a.side)
+        print(json.dumps(t), flush=True)
+        return
     tuning = json.load(open(a.tuning))["tuning"] if os.path.exists(a.tuning) else {}
     jobs = batch_workload(a.jobs, seed=4)
     with Engine(**tuning) as eng:
