@@ -123,14 +123,23 @@ def test_request_geometries_against_oracle(oracle, M, geometry, E_eV):
 
 
 def test_image_full_conservation(M):
-    """The full 256 x 256 x 300 image (19,660,800 primaries)."""
+    """The full 256 x 256 x 300 image (19,660,800 primaries), on the engine's
+    default settings (262,144 primaries per launch, geometry gathering chosen
+    by the scene) and again with every primary admitted at once and no
+    gathering: the same tallies."""
     from cudamonsel_amd import scenes, serving
     from cudamonsel_amd.abi import Engine
     sc = scenes.random_lines(seed=3)
     cen = serving.raster_centers(256, sc.beam_z)
+    cs = sc.monte.compileScene()
     with Engine() as eng:
-        sid = eng.upload(sc.monte.compileScene())
+        sid = eng.upload(cs)
         r = eng.result(eng.submit(sid, cen, 300, 500.0, 0.5e-9, 3), timeout_s=100)
+    with Engine(primaries_per_launch=1 << 30, geometry_batch=1) as eng:
+        sid = eng.upload(cs)
+        other = eng.result(eng.submit(sid, cen, 300, 500.0, 0.5e-9, 3), timeout_s=100)
+    assert r["stats"]["transport_launches"] > other["stats"]["transport_launches"]
+    _same(r, other)
     st = r["stats"]
     assert st["primaries"] == 256 * 256 * 300
     assert st["watchdog_drops"] == 0 and st["queue_overflow"] == 0
